@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""720p stereo pairs/s of plain PatchMatch on one MI355X.
+
+Workload (one step = one stereo pair, inputs resident in HBM): a seeded
+synthetic 1280x720 RGBA float32 pair with 24 slanted planar cells and
+disparities in 0-128 px; precompute (bilateral weights, gradients), random
+plane init of both views, 8 PatchMatch iterations (left then right view)
+with a 9x9 patch, LR check at 1 px (angle test off) and disparity
+conversion, all on the engine's stream.
+
+Prints one JSON line with the throughput, the roofline of the dominant
+kernel (k_propagate: FP32 VALU bound, algorithmic FLOPs from an exact tap
+count) and the CPU baseline (the oracle on the host cores, timed on a
+bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("720p stereo pairs/s at matched bad-1.0px; learned-init vs plain "
+          "PatchMatch")
+W, H, CELLS, DMAX, RADIUS, ITERS, SEED = 1280, 720, 24, 128.0, 4, 8, 0
+MAXDISP, MAXDEG = 1.0, 180.0
+# algorithmic FP32 work per valid tap (FMA = 2 FLOP):
+#  cost tap  = transform 4 + colour lerp 10 + gradient lerp 13 + colour term
+#              13 + sign term 6 + accumulate/exit 4                 = 50
+#  KLT tap   = 3 lerped colour/gradient samples + 3 cost terms + the
+#              outer-product accumulate                             = 150
+FLOP_PER_COST_TAP = 50
+FLOP_PER_KLT_TAP = 150
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(L, R, gt):
+    """The oracle (C, OpenMP on all host cores) on a bounded sample: the
+    full setup, one of the eight iterations and the post-processing of the
+    same pair; per-pair time = setup + 8 x iteration + post."""
+    import numpy as np
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    tl, tr = O.prepare(L, RADIUS), O.prepare(R, RADIUS)
+    sl = O.init(tl, tr, RADIUS, DMAX, 1)
+    sr = O.init(tr, tl, RADIUS, DMAX, 0)
+    t1 = time.perf_counter()
+    vb2 = O.viewbuf_empty(W, H)
+    sl, vb1 = O.propagate(tl, tr, RADIUS, DMAX, 1, 0, sl, vb2, sr)
+    sr, vb2 = O.propagate(tr, tl, RADIUS, DMAX, 0, 0, sr, vb1, sl)
+    t2 = time.perf_counter()
+    d = O.to_disparity(O.consistency(sl, sr, MAXDISP, MAXDEG))
+    t3 = time.perf_counter()
+    per_pair = (t1 - t0) + ITERS * (t2 - t1) + (t3 - t2)
+    del d, np
+    return {"value": 1.0 / per_pair, "unit": "pairs/s",
+            "cores": O.num_threads(), "kind": "port",
+            "sample": "1280x720 pair: setup + 1 of %d iterations + LR check "
+                      "timed (%.1f s); per pair = setup %.2f s + %d x "
+                      "iteration %.2f s + post %.3f s" %
+                      (ITERS, t3 - t0, t1 - t0, ITERS, t2 - t1, t3 - t2),
+            "ms_per_pair": per_pair * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the PatchMatch engine runs on a "
+              "single MI355X and has no multi-GPU path (--gpus %d given)"
+              % args.gpus, file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import cvt_amd
+    from cvt_amd import synth
+
+    log("bench: generating the 1280x720 pair")
+    L, R, gt = synth.make_pair(W, H, cells=CELLS, dmax=DMAX, channels=4,
+                               seed=SEED)
+    dev = torch.device("cuda:0")
+    dL = torch.from_numpy(L).to(dev)
+    dR = torch.from_numpy(R).to(dev)
+    dD = torch.empty((H, W), dtype=torch.float32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    ctx = cvt_amd.Context(radius=RADIUS, disparity_max=DMAX,
+                          stream=stream.cuda_stream)
+
+    def step(events=None):
+        ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
+        ctx.setup()
+        for i in range(ITERS):
+            if events is not None:
+                events[i][0].record(stream)
+            ctx.iterate(1)
+            if events is not None:
+                events[i][1].record(stream)
+        ctx.disparity_device(dD.data_ptr(), cvt_amd.DISPARITY_PP_LRCHECK,
+                             MAXDISP, MAXDEG)
+
+    with torch.cuda.stream(stream):
+        for i in range(args.warmup):
+            step()
+            log("bench: warmup step %d done" % (i + 1))
+        ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)]
+               for _ in range(ITERS)] for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(ev[k])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        log("bench: %d timed steps done" % args.steps)
+        disp = dD.cpu().numpy()
+        # per iteration = one k_propagate per view (+ two 8 B/pixel buffer
+        # clears, < 0.1 %)
+        iter_ms = [ev[k][i][0].elapsed_time(ev[k][i][1])
+                   for k in range(args.steps) for i in range(ITERS)]
+        prop_ms = float(np.mean(iter_ms)) / 2.0
+        # untimed counting pass: identical inputs, identical work
+        ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
+        ctx.setup()
+        cost_taps, klt_taps = ctx.iterate_counted(ITERS)
+        torch.cuda.synchronize()
+    ctx.close()
+
+    launches = 2 * ITERS
+    flop = (FLOP_PER_COST_TAP * cost_taps + FLOP_PER_KLT_TAP * klt_taps) \
+        / launches
+    achieved = flop / (prop_ms * 1e-3) / 1e12
+    elapsed = t1 - t0
+    value = args.steps / elapsed
+    bad1 = synth.bad_pixel_rate(disp, gt, 1.0)
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Voronoi slanted planes, Perlin texture)",
+        "config": {"workload": "plain PatchMatch, 1280x720 RGBA f32 pair, 24 "
+                               "planar cells, disparity 0-128 px, 9x9 patch, "
+                               "8 iterations, LR check 1 px",
+                   "variant": "plain", "width": W, "height": H,
+                   "radius": RADIUS, "iterations": ITERS,
+                   "disparity_max": DMAX, "seed": SEED},
+        "quality": {"bad_1px": bad1},
+        "roofline": {"bound": "valu", "kernel": "k_propagate",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None, "avg_launch_ms": prop_ms,
+                     "cost_taps_per_launch": cost_taps / launches,
+                     "klt_taps_per_launch": klt_taps / launches},
+    }
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle) on a bounded sample")
+        result["cpu_baseline"] = cpu_baseline(L, R, gt)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,11 @@
+"""cvt_amd -- MI355X-native PatchMatch stereo engine.
+
+Drop-in for the setup/iterate path of cvt::PMHuberStereo: hand-written HIP
+kernels for gfx950 behind the C ABI of include/pm_stereo.h, a C++ class
+(include/cvt_amd/PMHuberStereo.h) and this Python interface.
+"""
+from .stereo import (DISPARITY_PP_LRCHECK, DISPARITY_PP_LRCHECK_FILL,  # noqa
+                     DISPARITY_PP_NONE, Context, PMError, PMHuberStereo)
+
+__all__ = ["Context", "PMHuberStereo", "PMError", "DISPARITY_PP_NONE",
+           "DISPARITY_PP_LRCHECK", "DISPARITY_PP_LRCHECK_FILL"]

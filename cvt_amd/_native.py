@@ -1,0 +1,75 @@
+"""ctypes binding of ``libcvt_amd_pm.so`` (the C ABI of include/pm_stereo.h).
+
+The library is built in-tree by ``cvt_amd/build.py``.  There is no fallback:
+a missing library raises, and a host without a HIP device gets PM_ERR_HIP
+from ``pm_create``.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO_PATH = os.path.join(HERE, "libcvt_amd_pm.so")
+
+PM_OK, PM_ERR_ARG, PM_ERR_STATE, PM_ERR_HIP, PM_ERR_NOMEM = range(5)
+PM_PP_NONE, PM_PP_LRCHECK, PM_PP_LRCHECK_FILL = range(3)
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_u64 = ctypes.c_uint64
+
+
+class PmParams(ctypes.Structure):
+    _fields_ = [("radius", ctypes.c_int),
+                ("disparity_max", ctypes.c_float),
+                ("color_weight", ctypes.c_float),
+                ("theta", ctypes.c_float),
+                ("device", ctypes.c_int),
+                ("stream", ctypes.c_void_p)]
+
+
+# name -> (restype, argtypes); every symbol include/pm_stereo.h declares
+SIGNATURES = {
+    "pm_params_default": (None, [ctypes.POINTER(PmParams)]),
+    "pm_create": (_i, [ctypes.POINTER(PmParams), ctypes.POINTER(_p)]),
+    "pm_destroy": (None, [_p]),
+    "pm_last_error": (ctypes.c_char_p, [_p]),
+    "pm_set_images": (_i, [_p, _p, _p, _i, _i, _i, _i]),
+    "pm_set_images_device": (_i, [_p, _p, _p, _i, _i, _i, _i]),
+    "pm_setup": (_i, [_p]),
+    "pm_setup_prior": (_i, [_p, _p, _p]),
+    "pm_iterate": (_i, [_p, _i]),
+    "pm_iteration": (_i, [_p]),
+    "pm_get_planes": (_i, [_p, _i, _p]),
+    "pm_set_planes": (_i, [_p, _i, _p, _i]),
+    "pm_cost": (_i, [_p, _i, _p, _p]),
+    "pm_disparity": (_i, [_p, _i, _f, _f, _p, _p]),
+    "pm_disparity_device": (_i, [_p, _i, _f, _f, _p]),
+    "pm_normals": (_i, [_p, _i, _p]),
+    "pm_sync": (_i, [_p]),
+    "pm_get_texels": (_i, [_p, _i, _p]),
+    "pm_draws": (_i, [_p, _u64, _i, _p]),
+    "pm_klt": (_i, [_p, _i, _p, _i, _p]),
+    "pm_iterate_counted": (_i, [_p, _i, _p, _p]),
+    "pm_math_selftest": (_i, [_p, _p, _i, _p]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO_PATH):
+            raise ImportError(
+                "cvt_amd: native library %s is missing; build it with "
+                "`python cvt_amd/build.py` (or __graft_entry__.build())"
+                % SO_PATH)
+        L = ctypes.CDLL(SO_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib

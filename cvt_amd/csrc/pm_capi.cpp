@@ -1,0 +1,641 @@
+// pm_capi.cpp -- C ABI of the PatchMatch engine (include/pm_stereo.h).
+//
+// Host orchestration of cvt::PMHuberStereo's plain PatchMatch path
+// (cvt/vision/PMHuberStereo.cpp:349-404,406-581,583-644) over the HIP
+// kernels in pm_engine.hip.  Every device buffer is owned by the context
+// and sized for the current image pair; all work is queued on one stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "pm_engine.h"
+#include "pm_stereo.h"
+
+namespace {
+
+constexpr int kMaxRadius = 16;
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct pm_ctx {
+  pm_params prm{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+
+  pm::Geometry geo;
+  int channels = 0;
+  bool have_images = false;
+  bool ready = false;   // setup done
+  int iteration = 0;
+  int cur = 0;          // index of the current plane field per view
+  int last_pp = -1;     // post-processing of the last pm_disparity
+
+  DevBuf img[2];        // packed input images
+  DevBuf tex[2], sgn[2], wts[2];
+  DevBuf state[2][2];   // [view][ping-pong]
+  DevBuf vbuf[2];       // vbuf[0]: filled by the left view, vbuf[1]: right
+  DevBuf checked;       // LR-checked left field
+  DevBuf scratch4;      // W*H float4 (host transfers, normals)
+  DevBuf scratch4b;     // W*H float4
+  DevBuf scratch1;      // W*H float
+  DevBuf counters;      // 2 x u64 tap counters of the counting pass
+};
+
+namespace {
+
+int fail(pm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+int hip_fail(pm_ctx* c, hipError_t e, const char* what) {
+  return fail(c, e == hipErrorOutOfMemory ? PM_ERR_NOMEM : PM_ERR_HIP,
+              "%s: %s", what, hipGetErrorString(e));
+}
+
+#define PM_HIP(ctx, call)                                   \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #call);  \
+  } while (0)
+
+int ensure(pm_ctx* c, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.ptr) return PM_OK;
+  if (b.ptr) {
+    (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  hipError_t e = hipMalloc(&b.ptr, bytes);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    return hip_fail(c, e, "hipMalloc");
+  }
+  b.bytes = bytes;
+  return PM_OK;
+}
+
+void release(DevBuf& b) {
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+int use_device(pm_ctx* c) {
+  PM_HIP(c, hipSetDevice(c->prm.device));
+  return PM_OK;
+}
+
+pm::ViewBufs view_bufs(pm_ctx* c, int v) {
+  pm::ViewBufs b;
+  b.tex = static_cast<float4*>(c->tex[v].ptr);
+  b.sgn = static_cast<uint8_t*>(c->sgn[v].ptr);
+  b.wts = static_cast<float*>(c->wts[v].ptr);
+  return b;
+}
+
+float4* state_ptr(pm_ctx* c, int view, int which) {
+  return static_cast<float4*>(c->state[view][which].ptr);
+}
+
+int check_params(pm_ctx* c, const pm_params* p) {
+  if (!p) return fail(c, PM_ERR_ARG, "params is NULL");
+  if (p->radius < 1 || p->radius > kMaxRadius)
+    return fail(c, PM_ERR_ARG, "radius %d outside [1, %d]", p->radius,
+                kMaxRadius);
+  if (!(p->disparity_max > 0.0f) || !std::isfinite(p->disparity_max))
+    return fail(c, PM_ERR_ARG, "disparity_max must be finite and > 0");
+  if (!(p->color_weight >= 0.0f) || !std::isfinite(p->color_weight))
+    return fail(c, PM_ERR_ARG, "color_weight must be finite and >= 0");
+  if (!(p->theta >= 0.0f) || !std::isfinite(p->theta))
+    return fail(c, PM_ERR_ARG, "theta must be finite and >= 0");
+  return PM_OK;
+}
+
+int check_view(pm_ctx* c, int view) {
+  if (view != 0 && view != 1)
+    return fail(c, PM_ERR_ARG, "view must be 0 (left) or 1 (right)");
+  return PM_OK;
+}
+
+// buffers that depend on the pair's size
+int alloc_pair(pm_ctx* c, int W, int H, int channels) {
+  const size_t n = (size_t)W * H;
+  const int T = 2 * c->prm.radius + 1;
+  int rc;
+  for (int v = 0; v < 2; ++v) {
+    if ((rc = ensure(c, c->img[v], n * channels * sizeof(float)))) return rc;
+    if ((rc = ensure(c, c->tex[v], n * 2 * sizeof(float4)))) return rc;
+    if ((rc = ensure(c, c->sgn[v], n))) return rc;
+    if ((rc = ensure(c, c->wts[v], n * (size_t)T * T * sizeof(float))))
+      return rc;
+    for (int k = 0; k < 2; ++k)
+      if ((rc = ensure(c, c->state[v][k], n * sizeof(float4)))) return rc;
+    if ((rc = ensure(c, c->vbuf[v], n * sizeof(unsigned long long))))
+      return rc;
+  }
+  if ((rc = ensure(c, c->checked, n * sizeof(float4)))) return rc;
+  if ((rc = ensure(c, c->scratch4, n * sizeof(float4)))) return rc;
+  if ((rc = ensure(c, c->scratch4b, n * sizeof(float4)))) return rc;
+  if ((rc = ensure(c, c->scratch1, n * sizeof(float)))) return rc;
+  c->geo.W = W;
+  c->geo.H = H;
+  c->geo.R = c->prm.radius;
+  c->geo.npix = n;
+  c->geo.color_weight = c->prm.color_weight;
+  c->channels = channels;
+  return PM_OK;
+}
+
+int check_image_args(pm_ctx* c, const void* l, const void* r, int W, int H,
+                     int channels, int stride) {
+  if (!c) return PM_ERR_ARG;
+  if (!l || !r) return fail(c, PM_ERR_ARG, "image pointer is NULL");
+  if (W < 1 || H < 1) return fail(c, PM_ERR_ARG, "invalid size %dx%d", W, H);
+  if ((size_t)W * H > (size_t)1 << 28)
+    return fail(c, PM_ERR_ARG, "image of %dx%d pixels is too large", W, H);
+  if (channels != 1 && channels != 4)
+    return fail(c, PM_ERR_ARG, "channels must be 1 (GRAY) or 4 (RGBA)");
+  if (stride < W * channels)
+    return fail(c, PM_ERR_ARG, "stride %d < width*channels %d", stride,
+                W * channels);
+  return PM_OK;
+}
+
+int set_images(pm_ctx* c, const float* l, const float* r, int W, int H,
+               int channels, int stride, hipMemcpyKind kind) {
+  int rc = check_image_args(c, l, r, W, H, channels, stride);
+  if (rc) return rc;
+  if ((rc = use_device(c))) return rc;
+  if ((rc = alloc_pair(c, W, H, channels))) return rc;
+  const size_t row = (size_t)W * channels * sizeof(float);
+  PM_HIP(c, hipMemcpy2DAsync(c->img[0].ptr, row, l, (size_t)stride * 4, row,
+                             H, kind, c->stream));
+  PM_HIP(c, hipMemcpy2DAsync(c->img[1].ptr, row, r, (size_t)stride * 4, row,
+                             H, kind, c->stream));
+  if (kind == hipMemcpyHostToDevice) PM_HIP(c, hipStreamSynchronize(c->stream));
+  c->have_images = true;
+  c->ready = false;
+  c->last_pp = -1;
+  return PM_OK;
+}
+
+// bilateral weights + gradients of both views (PMHuberStereo.cpp:445-450)
+int prepare_views(pm_ctx* c) {
+  for (int v = 0; v < 2; ++v)
+    PM_HIP(c, pm::launch_prepare(c->stream, c->geo,
+                                 static_cast<const float*>(c->img[v].ptr),
+                                 c->channels, c->geo.W * c->channels,
+                                 view_bufs(c, v)));
+  return PM_OK;
+}
+
+int to_device(pm_ctx* c, void* dst, const void* src, size_t bytes) {
+  PM_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return PM_OK;
+}
+
+int to_host(pm_ctx* c, void* dst, const void* src, size_t bytes) {
+  PM_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  PM_HIP(c, hipStreamSynchronize(c->stream));
+  return PM_OK;
+}
+
+int require_images(pm_ctx* c) {
+  if (!c->have_images)
+    return fail(c, PM_ERR_STATE, "no images: call pm_set_images first");
+  return PM_OK;
+}
+
+int require_ready(pm_ctx* c) {
+  if (!c->ready) return fail(c, PM_ERR_STATE, "call pm_setup first");
+  return PM_OK;
+}
+
+// disparityMapAll's per-iteration sequence (PMHuberStereo.cpp:531-543):
+// clear viewbuf1, propagate L (reads viewbuf2, writes viewbuf1), clear
+// viewbuf2, propagate R (reads viewbuf1, writes viewbuf2)
+int one_iteration(pm_ctx* c, unsigned long long* counters = nullptr) {
+  const int cur = c->cur, nxt = 1 - c->cur;
+  unsigned long long* vbL = static_cast<unsigned long long*>(c->vbuf[0].ptr);
+  unsigned long long* vbR = static_cast<unsigned long long*>(c->vbuf[1].ptr);
+  pm::PropagateArgs a;
+  a.dmax = c->prm.disparity_max;
+  a.theta = c->prm.theta;
+  a.iter = c->iteration;
+
+  PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vbL));
+  a.self = view_bufs(c, 0);
+  a.other = view_bufs(c, 1);
+  a.old = state_ptr(c, 0, cur);
+  a.smooth = nullptr;
+  a.viewin = vbR;
+  a.src_other = state_ptr(c, 1, cur);
+  a.viewout = vbL;
+  a.out = state_ptr(c, 0, nxt);
+  a.lr = 1;
+  PM_HIP(c, pm::launch_propagate(c->stream, c->geo, a, counters));
+
+  PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vbR));
+  a.self = view_bufs(c, 1);
+  a.other = view_bufs(c, 0);
+  a.old = state_ptr(c, 1, cur);
+  a.viewin = vbL;
+  a.src_other = state_ptr(c, 0, nxt);
+  a.viewout = vbR;
+  a.out = state_ptr(c, 1, nxt);
+  a.lr = 0;
+  PM_HIP(c, pm::launch_propagate(c->stream, c->geo, a, counters));
+
+  c->cur = nxt;
+  c->iteration += 1;
+  return PM_OK;
+}
+
+int postprocess(pm_ctx* c, int pp, float maxdispdiff, float maxdeg,
+                float* d_out) {
+  if (pp != PM_PP_NONE && pp != PM_PP_LRCHECK && pp != PM_PP_LRCHECK_FILL)
+    return fail(c, PM_ERR_ARG, "unknown post-processing %d", pp);
+  if (!(maxdispdiff >= 0.0f) || !(maxdeg >= 0.0f))
+    return fail(c, PM_ERR_ARG, "maxdispdiff and maxdeg must be >= 0");
+  const float4* L = state_ptr(c, 0, c->cur);
+  const float4* R = state_ptr(c, 1, c->cur);
+  float4* chk = static_cast<float4*>(c->checked.ptr);
+  if (pp == PM_PP_NONE) {
+    PM_HIP(c, pm::launch_to_disparity(c->stream, c->geo, L, 1.0f, 1, d_out));
+  } else {
+    PM_HIP(c, pm::launch_consistency(c->stream, c->geo, L, R, maxdispdiff,
+                                     maxdeg, chk));
+    if (pp == PM_PP_LRCHECK)
+      PM_HIP(c, pm::launch_to_disparity(c->stream, c->geo, chk, 1.0f, 1,
+                                        d_out));
+    else
+      PM_HIP(c, pm::launch_fill_depthmap(c->stream, c->geo, chk, 1.0f, d_out));
+  }
+  c->last_pp = pp;
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pm_params_default(pm_params* p) {
+  if (!p) return;
+  p->radius = 5;
+  p->disparity_max = 64.0f;
+  p->color_weight = 12.0f;
+  p->theta = 0.0f;
+  p->device = 0;
+  p->stream = nullptr;
+}
+
+int pm_create(const pm_params* p, pm_ctx** out) {
+  if (!out) return PM_ERR_ARG;
+  *out = nullptr;
+  pm_ctx* c = new (std::nothrow) pm_ctx();
+  if (!c) return PM_ERR_NOMEM;
+  int rc = check_params(c, p);
+  if (rc) {
+    *out = c;  // caller may read pm_last_error, then pm_destroy
+    return rc;
+  }
+  c->prm = *p;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    *out = c;
+    return fail(c, PM_ERR_HIP, "no HIP device available (%s)",
+                e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+  }
+  if (p->device < 0 || p->device >= ndev) {
+    *out = c;
+    return fail(c, PM_ERR_ARG, "device %d outside [0, %d)", p->device, ndev);
+  }
+  *out = c;
+  if ((rc = use_device(c))) return rc;
+  if (p->stream) {
+    c->stream = static_cast<hipStream_t>(p->stream);
+  } else {
+    PM_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+  }
+  return PM_OK;
+}
+
+void pm_destroy(pm_ctx* c) {
+  if (!c) return;
+  if (c->stream || c->geo.npix) {
+    (void)hipSetDevice(c->prm.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+  }
+  for (int v = 0; v < 2; ++v) {
+    release(c->img[v]);
+    release(c->tex[v]);
+    release(c->sgn[v]);
+    release(c->wts[v]);
+    release(c->state[v][0]);
+    release(c->state[v][1]);
+    release(c->vbuf[v]);
+  }
+  release(c->checked);
+  release(c->scratch4);
+  release(c->scratch4b);
+  release(c->scratch1);
+  release(c->counters);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* pm_last_error(const pm_ctx* c) {
+  return c ? c->err.c_str() : "context is NULL";
+}
+
+int pm_set_images(pm_ctx* c, const float* l, const float* r, int W, int H,
+                  int channels, int stride) {
+  return set_images(c, l, r, W, H, channels, stride, hipMemcpyHostToDevice);
+}
+
+int pm_set_images_device(pm_ctx* c, const float* l, const float* r, int W,
+                         int H, int channels, int stride) {
+  return set_images(c, l, r, W, H, channels, stride, hipMemcpyDeviceToDevice);
+}
+
+int pm_setup(pm_ctx* c) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = require_images(c)) || (rc = use_device(c))) return rc;
+  if ((rc = prepare_views(c))) return rc;
+  PM_HIP(c, pm::launch_init(c->stream, c->geo, view_bufs(c, 0),
+                            view_bufs(c, 1), c->prm.disparity_max, 1,
+                            state_ptr(c, 0, 0)));
+  PM_HIP(c, pm::launch_init(c->stream, c->geo, view_bufs(c, 1),
+                            view_bufs(c, 0), c->prm.disparity_max, 0,
+                            state_ptr(c, 1, 0)));
+  PM_HIP(c, pm::launch_viewbuf_clear(
+                c->stream, c->geo,
+                static_cast<unsigned long long*>(c->vbuf[1].ptr)));
+  c->cur = 0;
+  c->iteration = 0;
+  c->ready = true;
+  c->last_pp = -1;
+  return PM_OK;
+}
+
+int pm_setup_prior(pm_ctx* c, const float* disparity, const float* normals) {
+  if (!c) return PM_ERR_ARG;
+  if (!disparity) return fail(c, PM_ERR_ARG, "disparity is NULL");
+  int rc;
+  if ((rc = require_images(c)) || (rc = use_device(c))) return rc;
+  const size_t n = c->geo.npix;
+  if ((rc = prepare_views(c))) return rc;
+  float* d_disp = static_cast<float*>(c->scratch1.ptr);
+  float4* d_norm = static_cast<float4*>(c->scratch4.ptr);
+  if ((rc = to_device(c, d_disp, disparity, n * sizeof(float)))) return rc;
+  if (normals) {
+    if ((rc = to_device(c, d_norm, normals, n * sizeof(float4)))) return rc;
+  } else {
+    PM_HIP(c, pm::launch_disparity_to_normal(c->stream, c->geo, d_disp,
+                                             d_norm));
+  }
+  PM_HIP(c, pm::launch_init_prior(c->stream, c->geo, view_bufs(c, 0),
+                                  view_bufs(c, 1), c->prm.disparity_max, 1,
+                                  d_disp, d_norm, state_ptr(c, 0, 0)));
+  PM_HIP(c, pm::launch_init(c->stream, c->geo, view_bufs(c, 1),
+                            view_bufs(c, 0), c->prm.disparity_max, 0,
+                            state_ptr(c, 1, 0)));
+  PM_HIP(c, pm::launch_viewbuf_clear(
+                c->stream, c->geo,
+                static_cast<unsigned long long*>(c->vbuf[1].ptr)));
+  PM_HIP(c, hipStreamSynchronize(c->stream));  // host inputs may be freed
+  c->cur = 0;
+  c->iteration = 0;
+  c->ready = true;
+  c->last_pp = -1;
+  return PM_OK;
+}
+
+int pm_iterate(pm_ctx* c, int n) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = require_ready(c)) || (rc = use_device(c))) return rc;
+  if (n < 0) return fail(c, PM_ERR_ARG, "negative iteration count %d", n);
+  for (int i = 0; i < n; ++i)
+    if ((rc = one_iteration(c))) return rc;
+  c->last_pp = -1;
+  return PM_OK;
+}
+
+int pm_iteration(const pm_ctx* c) { return c ? c->iteration : -1; }
+
+int pm_iterate_counted(pm_ctx* c, int n, uint64_t* cost_taps,
+                       uint64_t* klt_taps) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = require_ready(c)) || (rc = use_device(c))) return rc;
+  if (n < 0) return fail(c, PM_ERR_ARG, "negative iteration count %d", n);
+  if (!cost_taps || !klt_taps) return fail(c, PM_ERR_ARG, "NULL counter");
+  if ((rc = ensure(c, c->counters, 2 * sizeof(unsigned long long)))) return rc;
+  unsigned long long* d = static_cast<unsigned long long*>(c->counters.ptr);
+  const unsigned long long zero[2] = {0ull, 0ull};
+  if ((rc = to_device(c, d, zero, sizeof zero))) return rc;
+  for (int i = 0; i < n; ++i)
+    if ((rc = one_iteration(c, d))) return rc;
+  unsigned long long h[2];
+  if ((rc = to_host(c, h, d, sizeof h))) return rc;
+  *cost_taps = h[0];
+  *klt_taps = h[1];
+  c->last_pp = -1;
+  return PM_OK;
+}
+
+int pm_get_planes(pm_ctx* c, int view, float* out) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = check_view(c, view)) || (rc = require_ready(c)) ||
+      (rc = use_device(c)))
+    return rc;
+  if (!out) return fail(c, PM_ERR_ARG, "out is NULL");
+  return to_host(c, out, state_ptr(c, view, c->cur),
+                 c->geo.npix * sizeof(float4));
+}
+
+int pm_set_planes(pm_ctx* c, int view, const float* in, int iteration) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = check_view(c, view)) || (rc = require_ready(c)) ||
+      (rc = use_device(c)))
+    return rc;
+  if (!in) return fail(c, PM_ERR_ARG, "in is NULL");
+  if (iteration < 0) return fail(c, PM_ERR_ARG, "negative iteration");
+  if ((rc = to_device(c, state_ptr(c, view, c->cur), in,
+                      c->geo.npix * sizeof(float4))))
+    return rc;
+  // the buffer the next left step reads is the right field's scatter of
+  // the previous iteration (empty before the first): rebuild it so that a
+  // run resumes exactly from the two fields and the counter
+  unsigned long long* vbR = static_cast<unsigned long long*>(c->vbuf[1].ptr);
+  PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vbR));
+  if (iteration > 0)
+    PM_HIP(c, pm::launch_scatter(c->stream, c->geo, state_ptr(c, 1, c->cur), 0,
+                                 vbR));
+  PM_HIP(c, hipStreamSynchronize(c->stream));
+  c->iteration = iteration;
+  c->last_pp = -1;
+  return PM_OK;
+}
+
+int pm_cost(pm_ctx* c, int view, const float* planes, float* cost) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = check_view(c, view)) || (rc = require_ready(c)) ||
+      (rc = use_device(c)))
+    return rc;
+  if (!planes || !cost) return fail(c, PM_ERR_ARG, "NULL buffer");
+  float4* d_pl = static_cast<float4*>(c->scratch4.ptr);
+  float* d_cost = static_cast<float*>(c->scratch1.ptr);
+  if ((rc = to_device(c, d_pl, planes, c->geo.npix * sizeof(float4))))
+    return rc;
+  PM_HIP(c, pm::launch_cost_field(c->stream, c->geo, view_bufs(c, view),
+                                  view_bufs(c, 1 - view), d_pl, d_cost));
+  return to_host(c, cost, d_cost, c->geo.npix * sizeof(float));
+}
+
+int pm_disparity(pm_ctx* c, int pp, float maxdispdiff, float maxdeg,
+                 float* disparity, float* checked) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = require_ready(c)) || (rc = use_device(c))) return rc;
+  if (!disparity) return fail(c, PM_ERR_ARG, "disparity is NULL");
+  float* d_disp = static_cast<float*>(c->scratch1.ptr);
+  if ((rc = postprocess(c, pp, maxdispdiff, maxdeg, d_disp))) return rc;
+  if (checked) {
+    const void* src = pp == PM_PP_NONE ? (const void*)state_ptr(c, 0, c->cur)
+                                       : c->checked.ptr;
+    PM_HIP(c, hipMemcpyAsync(checked, src, c->geo.npix * sizeof(float4),
+                             hipMemcpyDeviceToHost, c->stream));
+  }
+  return to_host(c, disparity, d_disp, c->geo.npix * sizeof(float));
+}
+
+int pm_disparity_device(pm_ctx* c, int pp, float maxdispdiff, float maxdeg,
+                        float* d_disparity) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = require_ready(c)) || (rc = use_device(c))) return rc;
+  if (!d_disparity) return fail(c, PM_ERR_ARG, "d_disparity is NULL");
+  return postprocess(c, pp, maxdispdiff, maxdeg, d_disparity);
+}
+
+int pm_normals(pm_ctx* c, int pp, float* normals) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = require_ready(c)) || (rc = use_device(c))) return rc;
+  if (!normals) return fail(c, PM_ERR_ARG, "normals is NULL");
+  if (pp != PM_PP_NONE && pp != PM_PP_LRCHECK && pp != PM_PP_LRCHECK_FILL)
+    return fail(c, PM_ERR_ARG, "unknown post-processing %d", pp);
+  if (pp != PM_PP_NONE && c->last_pp != pp)
+    return fail(c, PM_ERR_STATE,
+                "call pm_disparity with the same post-processing first");
+  const float4* src = pp == PM_PP_NONE
+                          ? state_ptr(c, 0, c->cur)
+                          : static_cast<const float4*>(c->checked.ptr);
+  float4* d_n = static_cast<float4*>(c->scratch4.ptr);
+  PM_HIP(c, pm::launch_fill_normalmap(c->stream, c->geo, src, d_n));
+  return to_host(c, normals, d_n, c->geo.npix * sizeof(float4));
+}
+
+int pm_sync(pm_ctx* c) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = use_device(c))) return rc;
+  if (c->stream) PM_HIP(c, hipStreamSynchronize(c->stream));
+  return PM_OK;
+}
+
+int pm_get_texels(pm_ctx* c, int view, float* out) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = check_view(c, view)) || (rc = require_ready(c)) ||
+      (rc = use_device(c)))
+    return rc;
+  if (!out) return fail(c, PM_ERR_ARG, "out is NULL");
+  return to_host(c, out, c->tex[view].ptr, c->geo.npix * 2 * sizeof(float4));
+}
+
+int pm_draws(pm_ctx* c, uint64_t k, int n, float* out) {
+  if (!c) return PM_ERR_ARG;
+  if (n < 0 || n > (1 << 26)) return fail(c, PM_ERR_ARG, "n out of range");
+  if (!out) return fail(c, PM_ERR_ARG, "out is NULL");
+  if (n == 0) return PM_OK;
+  int rc;
+  if ((rc = use_device(c))) return rc;
+  DevBuf tmp;
+  if ((rc = ensure(c, tmp, (size_t)n * sizeof(float)))) return rc;
+  hipError_t e = pm::launch_draws(c->stream, k, n, static_cast<float*>(tmp.ptr));
+  if (e == hipSuccess) rc = to_host(c, out, tmp.ptr, (size_t)n * sizeof(float));
+  else rc = hip_fail(c, e, "launch_draws");
+  release(tmp);
+  return rc;
+}
+
+int pm_klt(pm_ctx* c, int view, const float* planes, int passes, float* out) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = check_view(c, view)) || (rc = require_ready(c)) ||
+      (rc = use_device(c)))
+    return rc;
+  if (!planes || !out) return fail(c, PM_ERR_ARG, "NULL buffer");
+  if (passes < 0 || passes > 64) return fail(c, PM_ERR_ARG, "passes out of range");
+  float4* d_in = static_cast<float4*>(c->scratch4.ptr);
+  float4* d_out = static_cast<float4*>(c->scratch4b.ptr);
+  if ((rc = to_device(c, d_in, planes, c->geo.npix * sizeof(float4))))
+    return rc;
+  PM_HIP(c, pm::launch_klt_field(c->stream, c->geo, view_bufs(c, view),
+                                 view_bufs(c, 1 - view), d_in, passes, d_out));
+  return to_host(c, out, d_out, c->geo.npix * sizeof(float4));
+}
+
+int pm_math_selftest(pm_ctx* c, const float* in, int n, float* out) {
+  if (!c) return PM_ERR_ARG;
+  if (n < 0 || n > (1 << 24)) return fail(c, PM_ERR_ARG, "n out of range");
+  if (!in || !out) return fail(c, PM_ERR_ARG, "NULL buffer");
+  if (n == 0) return PM_OK;
+  int rc;
+  if ((rc = use_device(c))) return rc;
+  DevBuf din, dout;
+  if ((rc = ensure(c, din, (size_t)n * sizeof(float))) ||
+      (rc = ensure(c, dout, (size_t)n * 5 * sizeof(float)))) {
+    release(din);
+    release(dout);
+    return rc;
+  }
+  rc = to_device(c, din.ptr, in, (size_t)n * sizeof(float));
+  if (!rc) {
+    hipError_t e = pm::launch_math(c->stream, static_cast<float*>(din.ptr), n,
+                                   static_cast<float*>(dout.ptr));
+    rc = e == hipSuccess ? to_host(c, out, dout.ptr, (size_t)n * 5 * sizeof(float))
+                         : hip_fail(c, e, "launch_math");
+  }
+  release(din);
+  release(dout);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// pm_engine.h -- host-side launchers of the PatchMatch kernels
+// (pm_engine.hip).  Internal to the library: the public boundary is the C
+// ABI in include/pm_stereo.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace pm {
+
+// Device buffers of one view.  A texel record is two float4:
+// tex[2p] = (r, g, b, bilw), tex[2p+1] = (gx, gy, gxy, gyx);
+// sgn[p] = sign bits of the gradient; wts[t * npix + p] = support weight of
+// tap t = (dy + r) * (2r + 1) + (dx + r) around pixel p (0 outside).
+struct ViewBufs {
+  float4* tex = nullptr;
+  uint8_t* sgn = nullptr;
+  float* wts = nullptr;
+};
+
+struct Geometry {
+  int W = 0, H = 0, R = 0;
+  size_t npix = 0;
+  float color_weight = 12.0f;
+};
+
+hipError_t launch_prepare(hipStream_t st, const Geometry& g, const float* img,
+                          int channels, int stride, ViewBufs v);
+
+hipError_t launch_init(hipStream_t st, const Geometry& g, ViewBufs self,
+                       ViewBufs other, float dmax, int lr, float4* states);
+
+hipError_t launch_disparity_to_normal(hipStream_t st, const Geometry& g,
+                                      const float* disp, float4* normals);
+
+hipError_t launch_init_prior(hipStream_t st, const Geometry& g, ViewBufs self,
+                             ViewBufs other, float dmax, int lr,
+                             const float* disp, const float4* normals,
+                             float4* states);
+
+hipError_t launch_viewbuf_clear(hipStream_t st, const Geometry& g,
+                                unsigned long long* vbuf);
+
+struct PropagateArgs {
+  ViewBufs self, other;
+  const float4* old = nullptr;
+  const float4* smooth = nullptr;       // may be null (cleared field)
+  const unsigned long long* viewin = nullptr;
+  const float4* src_other = nullptr;    // states the viewin keys refer to
+  unsigned long long* viewout = nullptr;
+  float4* out = nullptr;
+  float dmax = 0.0f, theta = 0.0f;
+  int lr = 1, iter = 0;
+};
+
+// counters (optional, 2 x u64): cost taps and KLT taps evaluated, counted
+// with the reference's per-tap early exit
+hipError_t launch_propagate(hipStream_t st, const Geometry& g,
+                            const PropagateArgs& a,
+                            unsigned long long* counters = nullptr);
+
+hipError_t launch_scatter(hipStream_t st, const Geometry& g,
+                          const float4* states, int lr,
+                          unsigned long long* viewout);
+
+hipError_t launch_cost_field(hipStream_t st, const Geometry& g, ViewBufs self,
+                             ViewBufs other, const float4* planes,
+                             float* cost);
+
+hipError_t launch_klt_field(hipStream_t st, const Geometry& g, ViewBufs self,
+                            ViewBufs other, const float4* planes, int passes,
+                            float4* out);
+
+hipError_t launch_consistency(hipStream_t st, const Geometry& g,
+                              const float4* left, const float4* right,
+                              float maxdispdiff, float maxdeg, float4* out);
+
+hipError_t launch_to_disparity(hipStream_t st, const Geometry& g,
+                               const float4* states, float scale, int lr,
+                               float* disp);
+
+hipError_t launch_fill_depthmap(hipStream_t st, const Geometry& g,
+                                const float4* states, float scale,
+                                float* disp);
+
+hipError_t launch_fill_normalmap(hipStream_t st, const Geometry& g,
+                                 const float4* states, float4* normals);
+
+hipError_t launch_draws(hipStream_t st, uint64_t k, int n, float* out);
+
+hipError_t launch_math(hipStream_t st, const float* in, int n, float* out);
+
+}  // namespace pm

@@ -1,0 +1,285 @@
+"""Python host interface of the MI355X PatchMatch stereo engine.
+
+``PMHuberStereo`` mirrors cvt's class (cvt/vision/PMHuberStereo.h:42-110):
+``disparityMap(left, right, patchsize, depthmax, iterations, pp,
+normalmap, initdisparity, initnormalmap)`` with the reference's defaults,
+plus the public setup / iterate / planes interface.  ``Context`` is the
+thin object over one C-ABI context (include/pm_stereo.h); every call goes
+to the HIP engine, there is no CPU path.
+
+Images are numpy float32 arrays, ``(H, W)`` for GRAY or ``(H, W, 4)`` for
+RGBA; plane fields are ``(H, W, 4)`` = (a, b, c, cost).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+from ._native import (PM_PP_LRCHECK, PM_PP_LRCHECK_FILL, PM_PP_NONE,  # noqa
+                      PmParams)
+
+DISPARITY_PP_NONE = PM_PP_NONE
+DISPARITY_PP_LRCHECK = PM_PP_LRCHECK
+DISPARITY_PP_LRCHECK_FILL = PM_PP_LRCHECK_FILL
+
+
+class PMError(RuntimeError):
+    """A failed engine call: ``code`` is the pm_status, the message the
+    engine's pm_last_error text."""
+
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _image(img):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    if img.ndim == 2:
+        C = 1
+    elif img.ndim == 3 and img.shape[2] == 4:
+        C = 4
+    else:
+        raise ValueError("image must be (H, W) GRAY or (H, W, 4) RGBA, got %s"
+                         % (img.shape,))
+    H, W = img.shape[:2]
+    return img, W, H, C
+
+
+class Context:
+    """One engine context (pm_create ... pm_destroy)."""
+
+    def __init__(self, radius=5, disparity_max=64.0, color_weight=12.0,
+                 theta=0.0, device=0, stream=None):
+        self._lib = _native.lib()
+        p = PmParams()
+        self._lib.pm_params_default(ctypes.byref(p))
+        p.radius = int(radius)
+        p.disparity_max = float(disparity_max)
+        p.color_weight = float(color_weight)
+        p.theta = float(theta)
+        p.device = int(device)
+        p.stream = stream
+        h = ctypes.c_void_p()
+        rc = self._lib.pm_create(ctypes.byref(p), ctypes.byref(h))
+        self._h = h
+        if rc != _native.PM_OK:
+            msg = self._lib.pm_last_error(h).decode() if h else "pm_create"
+            if h:
+                self._lib.pm_destroy(h)
+            self._h = None
+            raise PMError(rc, msg)
+        self.radius = p.radius
+        self.disparity_max = p.disparity_max
+        self.shape = None
+        self.channels = None
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if self._h:
+            self._lib.pm_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != _native.PM_OK:
+            raise PMError(rc, self._lib.pm_last_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- inputs ------------------------------------------------------------
+    def set_images(self, left, right):
+        left, W, H, C = _image(left)
+        right, W2, H2, C2 = _image(right)
+        if (W, H, C) != (W2, H2, C2):
+            raise ValueError("left/right images differ in size or format")
+        self._check(self._lib.pm_set_images(self._h, _ptr(left), _ptr(right),
+                                            W, H, C, W * C))
+        self.shape, self.channels = (H, W), C
+
+    def set_images_device(self, d_left, d_right, W, H, channels, stride=None):
+        """Device pointers (ints) of two W x H images in HBM."""
+        stride = W * channels if stride is None else stride
+        self._check(self._lib.pm_set_images_device(
+            self._h, ctypes.c_void_p(d_left), ctypes.c_void_p(d_right), W, H,
+            channels, stride))
+        self.shape, self.channels = (H, W), channels
+
+    def setup(self):
+        self._check(self._lib.pm_setup(self._h))
+
+    def setup_prior(self, disparity, normals=None):
+        H, W = self.shape
+        d = np.ascontiguousarray(disparity, np.float32)
+        assert d.shape == (H, W)
+        n = None
+        if normals is not None:
+            n = np.ascontiguousarray(normals, np.float32)
+            assert n.shape == (H, W, 4)
+        self._check(self._lib.pm_setup_prior(self._h, _ptr(d),
+                                             _ptr(n) if n is not None
+                                             else None))
+
+    # -- iterate -------------------------------------------------------------
+    def iterate(self, n=1):
+        self._check(self._lib.pm_iterate(self._h, int(n)))
+
+    @property
+    def iteration(self):
+        return self._lib.pm_iteration(self._h)
+
+    def planes(self, view=0):
+        H, W = self.shape
+        out = np.empty((H, W, 4), np.float32)
+        self._check(self._lib.pm_get_planes(self._h, int(view), _ptr(out)))
+        return out
+
+    def set_planes(self, view, planes, iteration):
+        H, W = self.shape
+        p = np.ascontiguousarray(planes, np.float32)
+        assert p.shape == (H, W, 4)
+        self._check(self._lib.pm_set_planes(self._h, int(view), _ptr(p),
+                                            int(iteration)))
+
+    def cost(self, view, planes):
+        H, W = self.shape
+        p = np.ascontiguousarray(planes, np.float32)
+        assert p.shape == (H, W, 4)
+        out = np.empty((H, W), np.float32)
+        self._check(self._lib.pm_cost(self._h, int(view), _ptr(p), _ptr(out)))
+        return out
+
+    # -- outputs -------------------------------------------------------------
+    def disparity(self, pp=PM_PP_LRCHECK, maxdispdiff=1.0, maxdeg=5.0,
+                  checked=False):
+        H, W = self.shape
+        d = np.empty((H, W), np.float32)
+        chk = np.empty((H, W, 4), np.float32) if checked else None
+        self._check(self._lib.pm_disparity(self._h, int(pp),
+                                           float(maxdispdiff), float(maxdeg),
+                                           _ptr(d),
+                                           _ptr(chk) if checked else None))
+        return (d, chk) if checked else d
+
+    def disparity_device(self, d_out, pp=PM_PP_LRCHECK, maxdispdiff=1.0,
+                         maxdeg=5.0):
+        self._check(self._lib.pm_disparity_device(
+            self._h, int(pp), float(maxdispdiff), float(maxdeg),
+            ctypes.c_void_p(d_out)))
+
+    def normals(self, pp=PM_PP_LRCHECK):
+        H, W = self.shape
+        out = np.empty((H, W, 4), np.float32)
+        self._check(self._lib.pm_normals(self._h, int(pp), _ptr(out)))
+        return out
+
+    def sync(self):
+        self._check(self._lib.pm_sync(self._h))
+
+    # -- diagnostics ---------------------------------------------------------
+    def texels(self, view=0):
+        H, W = self.shape
+        out = np.empty((H, W, 8), np.float32)
+        self._check(self._lib.pm_get_texels(self._h, int(view), _ptr(out)))
+        return out
+
+    def draws(self, k, n):
+        out = np.empty(int(n), np.float32)
+        self._check(self._lib.pm_draws(self._h, int(k), int(n), _ptr(out)))
+        return out
+
+    def klt(self, view, planes, passes=1):
+        H, W = self.shape
+        p = np.ascontiguousarray(planes, np.float32)
+        out = np.empty((H, W, 4), np.float32)
+        self._check(self._lib.pm_klt(self._h, int(view), _ptr(p),
+                                     int(passes), _ptr(out)))
+        return out
+
+    def iterate_counted(self, n=1):
+        """iterate(n) that returns (cost_taps, klt_taps) evaluated."""
+        a = ctypes.c_uint64()
+        b = ctypes.c_uint64()
+        self._check(self._lib.pm_iterate_counted(self._h, int(n),
+                                                 ctypes.byref(a),
+                                                 ctypes.byref(b)))
+        return a.value, b.value
+
+    def math_selftest(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty((x.size, 5), np.float32)
+        self._check(self._lib.pm_math_selftest(self._h, _ptr(x), x.size,
+                                               _ptr(out)))
+        return out
+
+
+class PMHuberStereo:
+    """cvt::PMHuberStereo (cvt/vision/PMHuberStereo.h:42-110), plain
+    PatchMatch form (theta = 0, no primal-dual smoothing)."""
+
+    def __init__(self, device=0):
+        self.device = device
+        self._ctx = None
+        self._cfg = None
+
+    def _context(self, patchsize, depthmax):
+        cfg = (int(patchsize), float(depthmax))
+        if self._ctx is None or self._cfg != cfg:
+            if self._ctx is not None:
+                self._ctx.close()
+            self._ctx = Context(radius=cfg[0], disparity_max=cfg[1],
+                                device=self.device)
+            self._cfg = cfg
+        return self._ctx
+
+    def setup(self, left, right, patchsize, depthmax, initdisparity=None,
+              initnormalmap=None):
+        ctx = self._context(patchsize, depthmax)
+        ctx.set_images(left, right)
+        if initdisparity is not None:
+            ctx.setup_prior(initdisparity, initnormalmap)
+        else:
+            ctx.setup()
+        return ctx
+
+    def iterate(self, n):
+        self._ctx.iterate(n)
+
+    def planes(self, view=0):
+        return self._ctx.planes(view)
+
+    def disparityMap(self, left, right, patchsize, depthmax, iterations,
+                     pp=DISPARITY_PP_LRCHECK, normalmap=False,
+                     initdisparity=None, initnormalmap=None):
+        """Returns the left disparity map (and the normal map when
+        ``normalmap`` is true).  Iterations are made odd like
+        PMHuberStereo.cpp:490; LR check at 1 px / 5 degrees
+        (PMHuberStereo.cpp:79-80)."""
+        ctx = self.setup(left, right, patchsize, depthmax, initdisparity,
+                         initnormalmap)
+        ctx.iterate(int(iterations) | 1)
+        d = ctx.disparity(pp, 1.0, 5.0)
+        if normalmap:
+            return d, ctx.normals(pp)
+        return d
+
+    def close(self):
+        if self._ctx is not None:
+            self._ctx.close()
+            self._ctx = None

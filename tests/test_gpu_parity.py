@@ -1,0 +1,301 @@
+"""Parity of the HIP engine with the oracle, through the C ABI.
+
+The engine and the oracle evaluate the same operation sequence (see the
+arithmetic contract in cvt_amd/csrc/pm_device.h and oracle/pm_oracle.h),
+so integer and index results must match exactly and float results are
+compared bit for bit; the north-star bar (per-pixel cost within 1e-4
+relative, final disparity within 0.5 px on >= 99.5% of pixels) is asserted
+as well so a deviation is reported against it.
+"""
+import numpy as np
+import pytest
+
+import cvt_amd
+
+pytestmark = pytest.mark.gpu
+
+COST_RTOL = 1e-4
+
+
+def same(a, b):
+    """bitwise equality of float arrays, NaN == NaN"""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32),
+                                                 b.view(np.uint32))
+
+
+def first_diff(a, b):
+    idx = np.argwhere(a.view(np.uint32) != b.view(np.uint32))
+    return None if idx.size == 0 else (tuple(idx[0]), a[tuple(idx[0])],
+                                       b[tuple(idx[0])], len(idx))
+
+
+def cost_close(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return np.all(np.abs(a - b) <= COST_RTOL * np.maximum(np.abs(b), 1e-6))
+
+
+@pytest.fixture(scope="module")
+def c1(synth):
+    L, R, gt = synth.make_pair(320, 240, cells=8, dmax=32.0, channels=1,
+                               seed=1)
+    return L, R, gt
+
+
+@pytest.fixture(scope="module")
+def rgba_small(synth):
+    L, R, gt = synth.make_pair(157, 83, cells=6, dmax=24.0, channels=4,
+                               seed=7)
+    return L, R, gt
+
+
+def test_math_bitexact(oracle):
+    with cvt_amd.Context(radius=1, disparity_max=8) as ctx:
+        xs = np.concatenate([np.linspace(-86.5, 0.0, 3001),
+                             np.linspace(0.0, 6.2832, 3001),
+                             np.linspace(-1.0, 1.0, 3001),
+                             np.random.default_rng(0).uniform(0, 50, 3001)])
+        xs = xs.astype(np.float32)
+        out = ctx.math_selftest(xs)
+    for i, x in enumerate(xs):
+        s, c = oracle.sincosf(x)
+        want = [oracle.expf(x), s, c]
+        got = out[i, :3]
+        assert same(got, np.array(want, np.float32)), (x, got, want)
+        if -1.0 <= x <= 1.0:
+            assert same(out[i, 3:4], np.float32([oracle.acospif(x)])), x
+        if x >= 0:
+            assert out[i, 4] == np.sqrt(np.float32(x)), x
+
+
+@pytest.mark.parametrize("k", [0, 14 * 921607 - 100, 2 ** 40 + 123])
+def test_draws_bitexact(oracle, k):
+    with cvt_amd.Context(radius=1, disparity_max=8) as ctx:
+        got = ctx.draws(k, 5003)
+    assert same(got, oracle.draws(k, 5003))
+
+
+@pytest.mark.parametrize("which,radius", [("c1", 5), ("rgba", 4)])
+def test_texels_bitexact(oracle, c1, rgba_small, which, radius):
+    L, R, _ = c1 if which == "c1" else rgba_small
+    with cvt_amd.Context(radius=radius, disparity_max=32) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for view, img in ((0, L), (1, R)):
+            got = ctx.texels(view)
+            want = oracle.prepare(img, radius)
+            assert same(got, want), first_diff(got, want)
+
+
+@pytest.mark.parametrize("which,radius,dmax", [("c1", 5, 32.0),
+                                               ("rgba", 4, 24.0)])
+def test_init_bitexact(oracle, c1, rgba_small, which, radius, dmax):
+    L, R, _ = c1 if which == "c1" else rgba_small
+    tl, tr = oracle.prepare(L, radius), oracle.prepare(R, radius)
+    with cvt_amd.Context(radius=radius, disparity_max=dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for view, (a, b, lr) in enumerate(((tl, tr, 1), (tr, tl, 0))):
+            want = oracle.init(a, b, radius, dmax, lr)
+            got = ctx.planes(view)
+            assert cost_close(got[..., 3], want[..., 3])
+            assert same(got, want), first_diff(got, want)
+
+
+def test_cost_field_bitexact(oracle, rgba_small):
+    L, R, _ = rgba_small
+    H, W = L.shape[:2]
+    r = 4
+    tl, tr = oracle.prepare(L, r), oracle.prepare(R, r)
+    rng = np.random.default_rng(4)
+    planes = np.zeros((H, W, 4), np.float32)
+    planes[..., 0] = 1 + rng.normal(0, 0.05, (H, W))
+    planes[..., 1] = rng.normal(0, 0.05, (H, W))
+    planes[..., 2] = -rng.uniform(0, 24, (H, W))
+    planes[::17, ::13, 0] = np.nan         # invalid planes
+    planes[5::19, 3::11, 2] = 1e4          # outside the image
+    with cvt_amd.Context(radius=r, disparity_max=24) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for view, (a, b) in enumerate(((tl, tr), (tr, tl))):
+            got = ctx.cost(view, planes)
+            want = oracle.cost_field(a, b, r, planes)
+            assert cost_close(got, want)
+            assert same(got, want), first_diff(got, want)
+
+
+def test_klt_bitexact(oracle, synth):
+    L, R, _ = synth.make_pair(48, 32, cells=3, dmax=10.0, channels=4, seed=3)
+    r = 3
+    tl, tr = oracle.prepare(L, r), oracle.prepare(R, r)
+    rng = np.random.default_rng(5)
+    H, W = 32, 48
+    planes = np.zeros((H, W, 4), np.float32)
+    planes[..., 0] = 1 + rng.normal(0, 0.02, (H, W))
+    planes[..., 1] = rng.normal(0, 0.02, (H, W))
+    planes[..., 2] = -rng.uniform(0, 10, (H, W))
+    with cvt_amd.Context(radius=r, disparity_max=10) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for passes in (1, 6):
+            got = ctx.klt(0, planes, passes)
+            want = np.zeros_like(planes)
+            for y in range(H):
+                for x in range(W):
+                    s = planes[y, x]
+                    for _ in range(passes):
+                        s = oracle.klt_at(tl, tr, r, x, y, s)
+                    want[y, x] = s
+            assert same(got, want), first_diff(got, want)
+
+
+def run_oracle_iterations(oracle, L, R, radius, dmax, iters):
+    tl, tr = oracle.prepare(L, radius), oracle.prepare(R, radius)
+    sl = oracle.init(tl, tr, radius, dmax, 1)
+    sr = oracle.init(tr, tl, radius, dmax, 0)
+    vb2 = oracle.viewbuf_empty(L.shape[1], L.shape[0])
+    fields = []
+    for it in range(iters):
+        sl, vb1 = oracle.propagate(tl, tr, radius, dmax, 1, it, sl, vb2, sr)
+        sr, vb2 = oracle.propagate(tr, tl, radius, dmax, 0, it, sr, vb1, sl)
+        fields.append((sl, sr))
+    return fields
+
+
+@pytest.mark.parametrize("which,radius,dmax,iters",
+                         [("c1", 5, 32.0, 3), ("rgba", 4, 24.0, 4)])
+def test_iterations_bitexact(oracle, c1, rgba_small, which, radius, dmax,
+                             iters):
+    L, R, _ = c1 if which == "c1" else rgba_small
+    want = run_oracle_iterations(oracle, L, R, radius, dmax, iters)
+    with cvt_amd.Context(radius=radius, disparity_max=dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for it in range(iters):
+            ctx.iterate(1)
+            for view in (0, 1):
+                got = ctx.planes(view)
+                w = want[it][view]
+                # north-star bar first, then bit equality
+                assert cost_close(got[..., 3], w[..., 3]), (it, view)
+                assert same(got, w), (it, view, first_diff(got, w))
+
+
+def test_pipeline_matches_oracle_run(oracle, c1):
+    """C1 configuration end to end: 320x240 gray, 11x11 patch, 3
+    iterations, LR check at 1 px (angle test off)."""
+    L, R, gt = c1
+    ref = oracle.run(L, R, 5, 32.0, 3, maxdisp=1.0, maxdeg=180.0)
+    with cvt_amd.Context(radius=5, disparity_max=32.0) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(3)
+        d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 180.0)
+        assert same(ctx.planes(0), ref["stateL"])
+        assert same(ctx.planes(1), ref["stateR"])
+    close = np.abs(d - ref["disp"]) <= 0.5
+    assert close.mean() >= 0.995
+    assert same(d, ref["disp"]), first_diff(d, ref["disp"])
+
+
+@pytest.mark.parametrize("pp", [0, 1, 2])
+def test_postprocessing_bitexact(oracle, rgba_small, pp):
+    L, R, _ = rgba_small
+    r, dmax = 4, 24.0
+    with cvt_amd.Context(radius=r, disparity_max=dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(3)
+        sl, sr = ctx.planes(0), ctx.planes(1)
+        d, chk = ctx.disparity(pp, 1.0, 5.0, checked=True)
+        nrm = ctx.normals(pp)
+    if pp == 0:
+        want_chk = sl
+        want_d = oracle.to_disparity(sl)
+    else:
+        want_chk = oracle.consistency(sl, sr, 1.0, 5.0)
+        want_d = (oracle.to_disparity(want_chk) if pp == 1
+                  else oracle.fill_depthmap(want_chk))
+    assert same(chk, want_chk), first_diff(chk, want_chk)
+    assert same(d, want_d), first_diff(d, want_d)
+    want_n = oracle.fill_normalmap(want_chk)
+    assert same(nrm, want_n), first_diff(nrm, want_n)
+
+
+def test_prior_init_bitexact(oracle, rgba_small):
+    L, R, gt = rgba_small
+    r, dmax = 4, 24.0
+    prior = gt.copy()
+    prior[::7, :] = 0.0                      # unknown rows -> random init
+    tl, tr = oracle.prepare(L, r), oracle.prepare(R, r)
+    normals = oracle.disparity_to_normal(prior)
+    with cvt_amd.Context(radius=r, disparity_max=dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup_prior(prior)                 # normals from the disparity
+        want = oracle.init_prior(tl, tr, r, dmax, 1, prior, normals)
+        assert same(ctx.planes(0), want), first_diff(ctx.planes(0), want)
+        assert same(ctx.planes(1), oracle.init(tr, tl, r, dmax, 0))
+        ctx.setup_prior(prior, normals)        # explicit normals
+        assert same(ctx.planes(0), want)
+
+
+def test_resume_from_fields(rgba_small):
+    L, R, _ = rgba_small
+    with cvt_amd.Context(radius=4, disparity_max=24.0) as a:
+        a.set_images(L, R)
+        a.setup()
+        a.iterate(2)
+        sl, sr = a.planes(0), a.planes(1)
+        a.iterate(2)
+        end = (a.planes(0), a.planes(1))
+    with cvt_amd.Context(radius=4, disparity_max=24.0) as b:
+        b.set_images(L, R)
+        b.setup()
+        b.set_planes(0, sl, 2)
+        b.set_planes(1, sr, 2)
+        b.iterate(2)
+        assert same(b.planes(0), end[0]) and same(b.planes(1), end[1])
+
+
+def test_counted_iteration_is_identical(rgba_small):
+    L, R, _ = rgba_small
+    out = []
+    for counted in (False, True):
+        with cvt_amd.Context(radius=4, disparity_max=24.0) as ctx:
+            ctx.set_images(L, R)
+            ctx.setup()
+            if counted:
+                taps, klt = ctx.iterate_counted(2)
+                H, W = L.shape[:2]
+                assert 0 < taps < 2 * 2 * H * W * 13 * 81
+                assert klt <= 2 * 2 * H * W * 6 * 81
+            else:
+                ctx.iterate(2)
+            out.append((ctx.planes(0), ctx.planes(1)))
+    assert same(out[0][0], out[1][0]) and same(out[0][1], out[1][1])
+
+
+def test_error_paths():
+    with pytest.raises(cvt_amd.PMError) as e:
+        cvt_amd.Context(radius=0, disparity_max=8)
+    assert e.value.code == 1
+    with cvt_amd.Context(radius=2, disparity_max=8) as ctx:
+        with pytest.raises(cvt_amd.PMError) as e:
+            ctx.setup()
+        assert e.value.code == 2
+        with pytest.raises(ValueError):
+            ctx.set_images(np.zeros((4, 5), np.float32),
+                           np.zeros((4, 6), np.float32))
+        ctx.set_images(np.zeros((4, 5), np.float32),
+                       np.zeros((4, 5), np.float32))
+        with pytest.raises(cvt_amd.PMError) as e:
+            ctx.iterate(1)
+        assert e.value.code == 2
+        ctx.setup()
+        with pytest.raises(cvt_amd.PMError) as e:
+            ctx.planes(2)
+        assert e.value.code == 1
+        with pytest.raises(cvt_amd.PMError):
+            ctx.disparity(pp=7)
