@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the engine's plain copy kernel over 512 "
+                         "MiB (calibration run for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PatchMatch engine runs on a "
@@ -87,6 +90,30 @@ def main():
 
     import cvt_amd
     from cvt_amd import synth
+
+    if args.copy_probe:
+        nbytes = 512 << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        stream = torch.cuda.Stream()
+        with cvt_amd.Context(radius=1, disparity_max=1.0,
+                             stream=stream.cuda_stream) as ctx:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            reps = 5
+            with torch.cuda.stream(stream):
+                e0.record(stream)
+                for _ in range(reps):
+                    ctx.copy_probe(src.data_ptr(), dst.data_ptr(), nbytes)
+                e1.record(stream)
+                torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        assert bool((dst == 1).all())
+        print(json.dumps({"copy_probe_bytes_read": nbytes,
+                          "copy_probe_bytes_written": nbytes,
+                          "avg_ms": ms,
+                          "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return 0
 
     log("bench: generating the 1280x720 pair")
     L, R, gt = synth.make_pair(W, H, cells=CELLS, dmax=DMAX, channels=4,

@@ -51,6 +51,7 @@ SIGNATURES = {
     "pm_draws": (_i, [_p, _u64, _i, _p]),
     "pm_klt": (_i, [_p, _i, _p, _i, _p]),
     "pm_iterate_counted": (_i, [_p, _i, _p, _p]),
+    "pm_copy_probe": (_i, [_p, _p, _p, ctypes.c_size_t]),
     "pm_math_selftest": (_i, [_p, _p, _i, _p]),
 }
 

@@ -612,6 +612,20 @@ int pm_klt(pm_ctx* c, int view, const float* planes, int passes, float* out) {
   return to_host(c, out, d_out, c->geo.npix * sizeof(float4));
 }
 
+int pm_copy_probe(pm_ctx* c, const float* d_src, float* d_dst, size_t bytes) {
+  if (!c) return PM_ERR_ARG;
+  if (!d_src || !d_dst) return fail(c, PM_ERR_ARG, "NULL buffer");
+  if (bytes % 16 != 0 || (uintptr_t)d_src % 16 || (uintptr_t)d_dst % 16)
+    return fail(c, PM_ERR_ARG, "copy probe needs 16-byte sizes and alignment");
+  int rc;
+  if ((rc = use_device(c))) return rc;
+  PM_HIP(c, pm::launch_copy_probe(c->stream,
+                                  reinterpret_cast<const float4*>(d_src),
+                                  reinterpret_cast<float4*>(d_dst),
+                                  bytes / 16));
+  return PM_OK;
+}
+
 int pm_math_selftest(pm_ctx* c, const float* in, int n, float* out) {
   if (!c) return PM_ERR_ARG;
   if (n < 0 || n > (1 << 24)) return fail(c, PM_ERR_ARG, "n out of range");

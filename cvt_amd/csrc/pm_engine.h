@@ -91,4 +91,7 @@ hipError_t launch_draws(hipStream_t st, uint64_t k, int n, float* out);
 
 hipError_t launch_math(hipStream_t st, const float* in, int n, float* out);
 
+hipError_t launch_copy_probe(hipStream_t st, const float4* src, float4* dst,
+                             size_t n);
+
 }  // namespace pm

@@ -688,6 +688,16 @@ k_draws(uint64_t k, int n, float* __restrict__ out) {
   for (int j = 0; j < 16 && i0 + j < n; ++j) out[i0 + j] = mwc_next(&s);
 }
 
+// plain streaming copy of n float4: the known-size workload that the HBM
+// byte counters are calibrated with
+__global__ void __launch_bounds__(256)
+k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
+             size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
 __global__ void __launch_bounds__(256)
 k_math(const float* __restrict__ in, int n, float* __restrict__ out) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -808,6 +818,11 @@ hipError_t launch_draws(hipStream_t st, uint64_t k, int n, float* out) {
   const int threads = (n + 15) / 16;
   PM_LAUNCH(k_draws, dim3((unsigned)((threads + 255) / 256)), dim3(256), st,
             k, n, out);
+}
+
+hipError_t launch_copy_probe(hipStream_t st, const float4* src, float4* dst,
+                             size_t n) {
+  PM_LAUNCH(k_copy_probe, dim3(4096), dim3(256), st, src, dst, n);
 }
 
 hipError_t launch_math(hipStream_t st, const float* in, int n, float* out) {

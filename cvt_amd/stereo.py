@@ -221,6 +221,11 @@ class Context:
                                                  ctypes.byref(b)))
         return a.value, b.value
 
+    def copy_probe(self, d_src, d_dst, nbytes):
+        self._check(self._lib.pm_copy_probe(self._h, ctypes.c_void_p(d_src),
+                                            ctypes.c_void_p(d_dst),
+                                            int(nbytes)))
+
     def math_selftest(self, x):
         x = np.ascontiguousarray(x, np.float32)
         out = np.empty((x.size, 5), np.float32)

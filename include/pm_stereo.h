@@ -156,6 +156,11 @@ int pm_klt(pm_ctx* ctx, int view, const float* planes, int passes,
  * views and the n iterations; results are identical to pm_iterate */
 int pm_iterate_counted(pm_ctx* ctx, int n, uint64_t* cost_taps,
                        uint64_t* klt_taps);
+/* stream-ordered plain copy of `bytes` (multiple of 16, 16-byte aligned
+ * device buffers): the known-size kernel HBM byte counters are
+ * calibrated against */
+int pm_copy_probe(pm_ctx* ctx, const float* d_src, float* d_dst,
+                  size_t bytes);
 /* device evaluation of the engine's scalar math for n inputs:
  * out[5*i .. 5*i+4] = exp, sin, cos, acospi, sqrt of in[i] */
 int pm_math_selftest(pm_ctx* ctx, const float* in, int n, float* out);
