@@ -42,7 +42,7 @@ struct pm_ctx {
   int last_pp = -1;     // post-processing of the last pm_disparity
 
   DevBuf img[2];        // packed input images
-  DevBuf tex[2], sgn[2], wts[2];
+  DevBuf tex[2], sgn[2];
   DevBuf state[2][2];   // [view][ping-pong]
   DevBuf vbuf[2];       // vbuf[0]: filled by the left view, vbuf[1]: right
   DevBuf checked;       // LR-checked left field
@@ -106,7 +106,6 @@ pm::ViewBufs view_bufs(pm_ctx* c, int v) {
   pm::ViewBufs b;
   b.tex = static_cast<float4*>(c->tex[v].ptr);
   b.sgn = static_cast<uint8_t*>(c->sgn[v].ptr);
-  b.wts = static_cast<float*>(c->wts[v].ptr);
   return b;
 }
 
@@ -137,14 +136,11 @@ int check_view(pm_ctx* c, int view) {
 // buffers that depend on the pair's size
 int alloc_pair(pm_ctx* c, int W, int H, int channels) {
   const size_t n = (size_t)W * H;
-  const int T = 2 * c->prm.radius + 1;
   int rc;
   for (int v = 0; v < 2; ++v) {
     if ((rc = ensure(c, c->img[v], n * channels * sizeof(float)))) return rc;
     if ((rc = ensure(c, c->tex[v], n * 2 * sizeof(float4)))) return rc;
     if ((rc = ensure(c, c->sgn[v], n))) return rc;
-    if ((rc = ensure(c, c->wts[v], n * (size_t)T * T * sizeof(float))))
-      return rc;
     for (int k = 0; k < 2; ++k)
       if ((rc = ensure(c, c->state[v][k], n * sizeof(float4)))) return rc;
     if ((rc = ensure(c, c->vbuf[v], n * sizeof(unsigned long long))))
@@ -349,7 +345,6 @@ void pm_destroy(pm_ctx* c) {
     release(c->img[v]);
     release(c->tex[v]);
     release(c->sgn[v]);
-    release(c->wts[v]);
     release(c->state[v][0]);
     release(c->state[v][1]);
     release(c->vbuf[v]);

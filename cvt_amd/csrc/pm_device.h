@@ -299,10 +299,29 @@ __device__ __forceinline__ float gray3(float x, float y, float z) {
   return (x * 0.2126f + y * 0.7152f) + z * 0.0722f;
 }
 
-// support weight exp(-pow(|dot(d, gray)|, 1) * cw), pmhstereo.cl:701
+// exp_f for x <= 0 without branches: the same operation sequence, the
+// x < -87 case selected afterwards (equal to exp_f for every x <= 0)
+__device__ __forceinline__ float exp_nonpos(float x) {
+  const float k = rintf(x * 0x1.715476p+0f);
+  float r = fmaf(-k, 0x1.62e400p-1f, x);
+  r = fmaf(-k, 0x1.7f7d1cp-20f, r);
+  float p = 0x1.a01a02p-13f;
+  p = fmaf(p, r, 0x1.6c16c2p-10f);
+  p = fmaf(p, r, 0x1.111112p-7f);
+  p = fmaf(p, r, 0x1.555556p-5f);
+  p = fmaf(p, r, 0x1.555556p-3f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  const float v = ldexpf(p, (int)k);
+  return x < -87.0f ? 0.0f : v;
+}
+
+// support weight exp(-pow(|dot(d, gray)|, 1) * cw), pmhstereo.cl:701; the
+// argument is <= 0 (cw >= 0 is checked at context creation)
 __device__ __forceinline__ float support_weight(float dx, float dy, float dz,
                                                 float cw) {
-  return exp_f(-fabsf(gray3(dx, dy, dz)) * cw);
+  return exp_nonpos(-fabsf(gray3(dx, dy, dz)) * cw);
 }
 
 __device__ __forceinline__ uint32_t signmask4(float4 g) {

@@ -11,12 +11,10 @@ namespace pm {
 
 // Device buffers of one view.  A texel record is two float4:
 // tex[2p] = (r, g, b, bilw), tex[2p+1] = (gx, gy, gxy, gyx);
-// sgn[p] = sign bits of the gradient; wts[t * npix + p] = support weight of
-// tap t = (dy + r) * (2r + 1) + (dx + r) around pixel p (0 outside).
+// sgn[p] = sign bits of the gradient (bit k = signbit of component k).
 struct ViewBufs {
   float4* tex = nullptr;
   uint8_t* sgn = nullptr;
-  float* wts = nullptr;
 };
 
 struct Geometry {

@@ -1,13 +1,24 @@
 // pm_engine.hip -- gfx950 kernels of the PatchMatch stereo engine.
 //
-// One work-item per pixel, 64 x 4 work-groups: a wavefront covers 64
-// consecutive pixels of one row, so per-tap loads of the weight planes and
-// of the tap's own texel row are contiguous across the wave, and the
-// other view's samples (same row, x' = a x + b y + c) are nearly
-// contiguous for coherent plane fields.
+// Work decomposition (kernels that evaluate plane costs):
+//  - a work-group is a 64 x 4 pixel block, each wavefront one row of 64
+//    consecutive pixels; blocks are renumbered so that each XCD walks one
+//    contiguous band of the image (its L2 then holds the rows the band
+//    reads);
+//  - the block stages its own-view tile (pixels plus an r halo: colour and
+//    gradient sign bits) in LDS once; support weights are recomputed per
+//    tap from that tile instead of being streamed from HBM;
+//  - the other view is sampled straight from its 32-byte texel records
+//    (two adjacent records per linear sample), which keeps a wavefront's
+//    loads within few cache lines of the vector L1.
+// The cost loops run wave-uniformly over patch rows (one image row per
+// wavefront) and every lane stays resident (no early return): lanes are
+// masked with flags, and kernels reach the block barrier with all threads.
 //
 // Per-pixel work follows cvt/cl/kernel/pmhstereo.cl (cited per kernel);
 // the arithmetic contract is in pm_device.h.
+#include <climits>
+
 #include "pm_device.h"
 #include "pm_engine.h"
 
@@ -52,173 +63,320 @@ __device__ __forceinline__ Mwc mwc_jump(uint64_t k) {
   return s;
 }
 
-constexpr int kBX = 64, kBY = 4;
+// ---------------------------------------------------------------------
+// block geometry and LDS layout
+// ---------------------------------------------------------------------
+constexpr int kTX = 64;  // block width = one wavefront
+constexpr int kTY = 4;   // wavefronts (rows) per block
+constexpr int kXcds = 8;
 
 __host__ __forceinline__ dim3 grid_for(int W, int H) {
-  return dim3((unsigned)((W + kBX - 1) / kBX), (unsigned)((H + kBY - 1) / kBY));
+  return dim3((unsigned)((W + kTX - 1) / kTX), (unsigned)((H + kTY - 1) / kTY));
 }
 
-struct ViewDev {
-  const float4* __restrict__ tex1;
-  const float4* __restrict__ tex2;
-  const uint8_t* __restrict__ sgn1;
-  const float* __restrict__ wts1;
-  int W, H, R;
-  size_t npix;
-};
+// LDS bytes of the block kernels: the own-view tile
+__host__ __forceinline__ size_t block_lds_bytes(int R) {
+  return (size_t)(kTX + 2 * R) * (kTY + 2 * R) * sizeof(float4);
+}
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
-// Linear sample of a texel row at column u (CLK_FILTER_LINEAR with
-// CLK_ADDRESS_CLAMP_TO_EDGE, pmhstereo.cl:71): colour rgb and gradient.
+// Work-group (bx, by) of the image for the hardware work-group id.
+// Consecutive work-groups are dispatched round-robin over the XCDs; this
+// numbering gives XCD k the k-th contiguous eighth of the blocks.
+__device__ __forceinline__ void xcd_block(int* bx, int* by) {
+  const int nbx = (int)gridDim.x;
+  const int n = nbx * (int)gridDim.y;
+  const int hw = (int)blockIdx.y * nbx + (int)blockIdx.x;
+  const int xcd = hw % kXcds, j = hw / kXcds;
+  const int per = n / kXcds, rem = n % kXcds;
+  const int logical = xcd * per + min(xcd, rem) + j;
+  *bx = logical % nbx;
+  *by = logical / nbx;
+}
+
+// Everything one lane needs to evaluate planes for its pixel.
+struct Lane {
+  int W, H, R;
+  float Wf, cw;
+  int x, y;      // y is the same for the whole wavefront
+  bool valid;
+  float xf, yf;
+  size_t p;
+  float4 tc;     // own pixel colour
+  float bilw;    // sum of its support weights
+  const float4* trow0;  // LDS tile at this lane: tap (dx, dy) is
+                        // trow0[(dy + R) * TW + dx + R]
+  int TW;
+  const float4* __restrict__ tex2;  // other view's texel records (global)
+};
+
+// Block prologue shared by the cost-evaluating kernels: stages the own-view
+// tile (r, g, b, gradient sign bits) and returns the lane context.  Must be
+// reached by every thread of the block.
+__device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
+                            const ViewBufs& other, float4* smem) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int R = g.R, W = g.W, H = g.H;
+  const int TW = kTX + 2 * R, TH = kTY + 2 * R;
+  const int x0 = bx * kTX, y0 = by * kTY;
+  const int tid = threadIdx.y * kTX + threadIdx.x;
+  for (int i = tid; i < TW * TH; i += kTX * kTY) {
+    const int gx = x0 - R + i % TW, gy = y0 - R + i / TW;
+    float4 v = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+      const size_t q = (size_t)gy * W + gx;
+      const float4 a = self.tex[2 * q];
+      v = mk4(a.x, a.y, a.z, __uint_as_float((uint32_t)self.sgn[q]));
+    }
+    smem[i] = v;
+  }
+  __syncthreads();
+  // the row index is uniform per wavefront: keep it scalar
+  const int ly = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  Lane L;
+  L.W = W;
+  L.H = H;
+  L.R = R;
+  L.Wf = (float)W;
+  L.cw = g.color_weight;
+  L.x = x0 + (int)threadIdx.x;
+  L.y = y0 + ly;
+  L.valid = L.x < W && L.y < H;
+  L.xf = (float)L.x;
+  L.yf = (float)L.y;
+  L.p = L.valid ? (size_t)L.y * W + L.x : 0;
+  L.TW = TW;
+  L.trow0 = smem + ly * TW + threadIdx.x;
+  L.tex2 = other.tex;
+  L.tc = L.trow0[R * TW + R];
+  L.bilw = L.valid ? self.tex[2 * L.p].w : 1.0f;
+  return L;
+}
+
+// Linear sample (CLK_FILTER_LINEAR, CLK_ADDRESS_CLAMP_TO_EDGE,
+// pmhstereo.cl:71) of the other view at column u of a texel row:
+// t0 + a*(t1 - t0).  Adjacent 32-byte texel records keep a wavefront's
+// loads within few cache lines.
 struct Sample {
   float r, g, b;
   float4 grad;
 };
 
-__device__ __forceinline__ Sample sample_row(const float4* __restrict__ row,
-                                             int W, float u) {
-  u = fmaxf(fminf(u, (float)W), -1.0f);
-  const float fl = floorf(u);
-  const float a = u - fl;
-  int i0 = (int)fl;
-  int i1 = i0 + 1;
-  i0 = clampi(i0, 0, W - 1);
-  i1 = clampi(i1, 0, W - 1);
-  const float4 A0 = row[2 * i0], G0 = row[2 * i0 + 1];
-  const float4 A1 = row[2 * i1], G1 = row[2 * i1 + 1];
+struct Texel {
+  float4 A, G;  // (r, g, b, bilw), gradient
+};
+
+__device__ __forceinline__ Texel texel(const float4* __restrict__ srow,
+                                       uint32_t i) {
+  Texel t;
+  t.A = srow[2 * i];
+  t.G = srow[2 * i + 1];
+  return t;
+}
+
+__device__ __forceinline__ Sample lerp_texels(const Texel& t0,
+                                              const Texel& t1, float a) {
   Sample s;
-  s.r = fmaf(a, A1.x - A0.x, A0.x);
-  s.g = fmaf(a, A1.y - A0.y, A0.y);
-  s.b = fmaf(a, A1.z - A0.z, A0.z);
-  s.grad.x = fmaf(a, G1.x - G0.x, G0.x);
-  s.grad.y = fmaf(a, G1.y - G0.y, G0.y);
-  s.grad.z = fmaf(a, G1.z - G0.z, G0.z);
-  s.grad.w = fmaf(a, G1.w - G0.w, G0.w);
+  s.r = fmaf(a, t1.A.x - t0.A.x, t0.A.x);
+  s.g = fmaf(a, t1.A.y - t0.A.y, t0.A.y);
+  s.b = fmaf(a, t1.A.z - t0.A.z, t0.A.z);
+  s.grad.x = fmaf(a, t1.G.x - t0.G.x, t0.G.x);
+  s.grad.y = fmaf(a, t1.G.y - t0.G.y, t0.G.y);
+  s.grad.z = fmaf(a, t1.G.z - t0.G.z, t0.G.z);
+  s.grad.w = fmaf(a, t1.G.w - t0.G.w, t0.G.w);
   return s;
 }
 
+// u clamped to [-1, W] like the sampler's coordinate, then the texel pair
+struct Coord {
+  float a;
+  int f;  // floor of the clamped coordinate, in [-1, W]
+};
+
+__device__ __forceinline__ Coord coord_of(const Lane& L, float u) {
+  u = fmaxf(fminf(u, L.Wf), -1.0f);
+  const float fl = floorf(u);
+  Coord c;
+  c.a = u - fl;
+  c.f = (int)fl;
+  return c;
+}
+
+__device__ __forceinline__ Sample sample(const Lane& L,
+                                         const float4* __restrict__ srow,
+                                         float u) {
+  const Coord c = coord_of(L, u);
+  const uint32_t i0 = (uint32_t)clampi(c.f, 0, L.W - 1);
+  const uint32_t i1 = (uint32_t)clampi(c.f + 1, 0, L.W - 1);
+  return lerp_texels(texel(srow, i0), texel(srow, i1), c.a);
+}
+
+// sample() for 0 <= u < W, where the coordinate and the first index need
+// no clamping
+__device__ __forceinline__ Sample sample_inside(
+    const Lane& L, const float4* __restrict__ srow, float u) {
+  const float fl = floorf(u);
+  const uint32_t i0 = (uint32_t)(int)fl;
+  const uint32_t i1 = min(i0 + 1u, (uint32_t)(L.W - 1));
+  return lerp_texels(texel(srow, i0), texel(srow, i1), u - fl);
+}
+
+// gradient sign mismatches between sign bits m1 and the signs of g
+__device__ __forceinline__ int sign_mismatches(uint32_t m1, float4 g) {
+  return __popc(m1 ^ signmask4(g));
+}
+
 // ---------------------------------------------------------------------
-// patch_eval_census (pmhstereo.cl:661-736).  The reference tests
+// patch_eval_census (pmhstereo.cl:661-736), one plane per lane (lanes with
+// active == false idle), patch rows wave-uniform.  The reference tests
 // census/bilw >= mincost after every tap; the census terms are >= 0 and
-// accumulate monotonically, so testing at the end of each row (and after
-// the last tap) returns 1.0 exactly when the per-tap test would.
-// kCount: also count the taps the reference's per-tap test evaluates
-// (the algorithmic work; used by an untimed counting pass only).
+// accumulate monotonically, so testing at the end of each row returns 1.0
+// exactly when the per-tap test would.
+// kCount: count the taps the reference's per-tap test evaluates (the
+// algorithmic work; used by an untimed counting pass only).
 // ---------------------------------------------------------------------
-template <bool kCount = false>
-__device__ float cost_eval(const ViewDev& v, int x, int y, float4 s,
-                           float mincost, uint32_t* ntaps = nullptr) {
-  const float xf = (float)x, yf = (float)y;
-  const float Wf = (float)v.W;
-  const float xp = transform(s, xf, yf);
-  if (!(finite_f(s.x) && finite_f(s.y) && finite_f(s.z)) ||
-      to_normal(s).z < kNormalZMin || xp < 0.0f || xp >= Wf)
-    return 1e5f;
-  const int R = v.R, W = v.W, T = 2 * R + 1;
-  const size_t p = (size_t)y * W + x;
-  const float bilw = v.tex1[2 * p].w;
-  const int dy0 = max(-R, -y), dy1 = min(R, v.H - 1 - y);
-  const int dx0 = max(-R, -x), dx1 = min(R, W - 1 - x);
+template <bool kCount>
+__device__ float cost_eval(const Lane& L, float4 s, float mincost,
+                           bool active, uint32_t* ntaps) {
+  const int R = L.R, W = L.W;
+  bool live = active && L.valid;
+  float res = 0.0f;
+  if (live) {
+    const float xp = transform(s, L.xf, L.yf);
+    if (!(finite_f(s.x) && finite_f(s.y) && finite_f(s.z)) ||
+        to_normal(s).z < kNormalZMin || xp < 0.0f || xp >= L.Wf) {
+      res = 1e5f;
+      live = false;
+    }
+  }
+  const int dx0 = max(-R, -L.x), dx1 = min(R, W - 1 - L.x);
   float census = 0.0f, wsum = 0.0f;
-  for (int dy = dy0; dy <= dy1; ++dy) {
-    const int qy = y + dy;
+  for (int dy = -R; dy <= R; ++dy) {
+    if (!__any(live)) break;
+    const int qy = L.y + dy;  // uniform
+    if (qy < 0 || qy >= L.H) continue;
+    if (!live) continue;
     const float rowc = fmaf(s.y, (float)qy, s.z);
-    const float4* __restrict__ row1 = v.tex1 + 2 * (size_t)qy * W;
-    const float4* __restrict__ row2 = v.tex2 + 2 * (size_t)qy * W;
-    const uint8_t* __restrict__ srow = v.sgn1 + (size_t)qy * W;
-    const float* __restrict__ wrow = v.wts1 + (size_t)((dy + R) * T) * v.npix + p;
+    const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * W;
+    const float4* trow = L.trow0 + (dy + R) * L.TW + R;
     for (int dx = dx0; dx <= dx1; ++dx) {
-      const int qx = x + dx;
-      const float xq = fmaf(s.x, (float)qx, rowc);
-      if (xq < 0.0f || xq >= Wf) continue;
-      const float w = wrow[(size_t)(dx + R) * v.npix];
-      const float4 t1 = row1[2 * qx];
-      const uint32_t m1 = srow[qx];
-      const Sample s2 = sample_row(row2, W, xq);
+      const float xq = fmaf(s.x, (float)(L.x + dx), rowc);
+      if (xq < 0.0f || xq >= L.Wf) continue;
+      const float4 t1 = trow[dx];
+      const uint32_t m1 = __float_as_uint(t1.w);
+      const float w = support_weight(t1.x - L.tc.x, t1.y - L.tc.y,
+                                     t1.z - L.tc.z, L.cw);
+      const Sample s2 = sample_inside(L, srow, xq);
       const float c0 = fminf(fabsf(s2.r - t1.x), 0.05f) * 20.0f;
       const float c1 = fminf(fabsf(s2.g - t1.y), 0.05f) * 20.0f;
       const float c2 = fminf(fabsf(s2.b - t1.z), 0.05f) * 20.0f;
       const float cad = 0.33f * ((c0 + c1) + c2);
-      float cost = 0.125f * cad;
-      const int nd = __popc(m1 ^ signmask4(s2.grad));
-      const float cc = 0.25f * (float)nd;
-      cost += 0.875f * cc;
+      // cost = 0.125 cad, then cost += 0.875 * (0.25 n): both products
+      // are exact, so one fma with 0.21875 n rounds the same sum
+      const float cost = fmaf(0.21875f, (float)sign_mismatches(m1, s2.grad),
+                              0.125f * cad);
       census = fmaf(w, cost, census);
       wsum += w;
       if (kCount) {
         ++*ntaps;
-        if (census / bilw >= mincost) return 1.0f;
+        if (census / L.bilw >= mincost) break;
       }
     }
-    if (census / bilw >= mincost) return 1.0f;
+    if (census / L.bilw >= mincost) {
+      res = 1.0f;
+      live = false;
+    }
   }
-  if (wsum <= 1.0f) return 1.1f;
-  return census / wsum;
+  if (live) res = (wsum <= 1.0f) ? 1.1f : census / wsum;
+  return res;
 }
 
 // one census term of patch_census_klt2 (pmhstereo.cl:360-392); v0..v2 is
-// the centred colour I1(q) - I1(p) the reference compares against
-__device__ __forceinline__ float klt_term(const float4* __restrict__ row2,
-                                          int W, float u, float v0, float v1,
-                                          float v2, uint32_t ga, float w1) {
-  const Sample s = sample_row(row2, W, u);
+// the centred colour I1(q) - I1(p) the reference compares against;
+// k1 = (w1 * alpha) * 0.33, k2 = w1 * (1 - alpha)
+__device__ __forceinline__ float klt_term(const Sample& s, float v0, float v1,
+                                          float v2, uint32_t ga, float k1,
+                                          float k2) {
   const float c0 = fminf(fabsf(s.r - v0), 0.05f) * 20.0f;
   const float c1 = fminf(fabsf(s.g - v1), 0.05f) * 20.0f;
   const float c2 = fminf(fabsf(s.b - v2), 0.05f) * 20.0f;
-  const float C = ((w1 * 0.125f) * 0.33f) * ((c0 + c1) + c2);
-  const int nd = __popc(ga ^ signmask4(s.grad));
+  const float C = k1 * ((c0 + c1) + c2);
   // C += w1 * (1 - alpha) * 0.25 * n, 0.25 a double literal
   // (pmhstereo.cl:370): exact product, one rounding
-  return fmaf(w1 * 0.875f, 0.25f * (float)nd, C);
+  return fmaf(k2, 0.25f * (float)sign_mismatches(ga, s.grad), C);
 }
 
 // patch_census_klt2 (pmhstereo.cl:326-410) + mat3f_lusolve
-// (cvt/cl/kernel/Matrix3.cl:286-311)
-__device__ float4 klt_pass(const ViewDev& v, int x, int y, float4 s) {
-  const int R = v.R, W = v.W, T = 2 * R + 1;
-  const size_t p = (size_t)y * W + x;
-  const float4 tc = v.tex1[2 * p];
+// (cvt/cl/kernel/Matrix3.cl:286-311), one plane per lane
+__device__ float4 klt_pass(const Lane& L, float4 s) {
+  const int R = L.R, W = L.W;
   float J0 = 0.0f, J1 = 0.0f, J2 = 0.0f;
   float H00 = 0.0f, H01 = 0.0f, H02 = 0.0f, H11 = 0.0f, H12 = 0.0f,
         H22 = 0.0f;
-  const int dy0 = max(-R, -y), dy1 = min(R, v.H - 1 - y);
   // taps need 1 <= x+dx < W-1 (the test is on the tap, pmhstereo.cl:358)
-  const int dx0 = max(-R, 1 - x), dx1 = min(R, W - 2 - x);
-  for (int dy = dy0; dy <= dy1; ++dy) {
-    const int qy = y + dy;
-    const float py = (float)qy;
-    const float rowc = fmaf(s.y, py, s.z);
-    const float4* __restrict__ row1 = v.tex1 + 2 * (size_t)qy * W;
-    const float4* __restrict__ row2 = v.tex2 + 2 * (size_t)qy * W;
-    const uint8_t* __restrict__ srow = v.sgn1 + (size_t)qy * W;
-    const float* __restrict__ wrow = v.wts1 + (size_t)((dy + R) * T) * v.npix + p;
-    for (int dx = dx0; dx <= dx1; ++dx) {
-      const int qx = x + dx;
-      const float px = (float)qx;
-      const float4 t1 = row1[2 * qx];
-      const float v0 = t1.x - tc.x, v1 = t1.y - tc.y, v2 = t1.z - tc.z;
-      const uint32_t ga = srow[qx];
-      const float w1 = wrow[(size_t)(dx + R) * v.npix];
-      const float xq = fmaf(s.x, px, rowc);
-      const float C = klt_term(row2, W, xq, v0, v1, v2, ga, w1);
-      const float Cxp = klt_term(row2, W, xq + 1.0f, v0, v1, v2, ga, w1);
-      const float Cxm = klt_term(row2, W, xq - 1.0f, v0, v1, v2, ga, w1);
-      const float curv = (Cxp + Cxm) - 2.0f * C;
-      if (curv > 0.0f) {
-        const float gj = (w1 * (Cxm - Cxp)) * 0.5f;
-        J0 = J0 + gj * px;
-        J1 = J1 + gj * py;
-        J2 = J2 + gj * 1.0f;
-        const float h = w1 * curv;
-        H00 = H00 + h * (px * px);
-        H01 = H01 + h * (px * py);
-        H02 = H02 + h * (px * 1.0f);
-        H11 = H11 + h * (py * py);
-        H12 = H12 + h * (py * 1.0f);
-        H22 = H22 + h * (1.0f * 1.0f);
+  const int dx0 = max(-R, 1 - L.x), dx1 = min(R, W - 2 - L.x);
+  if (L.valid) {
+    for (int dy = -R; dy <= R; ++dy) {
+      const int qy = L.y + dy;  // uniform
+      if (qy < 0 || qy >= L.H) continue;
+      const float py = (float)qy;
+      const float rowc = fmaf(s.y, py, s.z);
+      const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * W;
+      const float4* trow = L.trow0 + (dy + R) * L.TW + R;
+      for (int dx = dx0; dx <= dx1; ++dx) {
+        const float px = (float)(L.x + dx);
+        const float4 t1 = trow[dx];
+        const float v0 = t1.x - L.tc.x, v1 = t1.y - L.tc.y,
+                    v2 = t1.z - L.tc.z;
+        const uint32_t ga = __float_as_uint(t1.w);
+        const float w1 = support_weight(v0, v1, v2, L.cw);
+        const float k1 = (w1 * 0.125f) * 0.33f, k2 = w1 * 0.875f;
+        const float xq = fmaf(s.x, px, rowc);
+        const Coord cm = coord_of(L, xq - 1.0f);
+        const Coord c0 = coord_of(L, xq);
+        const Coord cp = coord_of(L, xq + 1.0f);
+        Sample sm, s0, sp;
+        if (__all(c0.f == cm.f + 1 && cp.f == cm.f + 2)) {
+          // consecutive floors: the three samples share four texels
+          const Texel t0 = texel(srow, (uint32_t)clampi(cm.f, 0, W - 1));
+          const Texel t1 = texel(srow, (uint32_t)clampi(cm.f + 1, 0, W - 1));
+          const Texel t2 = texel(srow, (uint32_t)clampi(cm.f + 2, 0, W - 1));
+          const Texel t3 = texel(srow, (uint32_t)clampi(cm.f + 3, 0, W - 1));
+          sm = lerp_texels(t0, t1, cm.a);
+          s0 = lerp_texels(t1, t2, c0.a);
+          sp = lerp_texels(t2, t3, cp.a);
+        } else {
+          sm = lerp_texels(texel(srow, (uint32_t)clampi(cm.f, 0, W - 1)),
+                           texel(srow, (uint32_t)clampi(cm.f + 1, 0, W - 1)),
+                           cm.a);
+          s0 = lerp_texels(texel(srow, (uint32_t)clampi(c0.f, 0, W - 1)),
+                           texel(srow, (uint32_t)clampi(c0.f + 1, 0, W - 1)),
+                           c0.a);
+          sp = lerp_texels(texel(srow, (uint32_t)clampi(cp.f, 0, W - 1)),
+                           texel(srow, (uint32_t)clampi(cp.f + 1, 0, W - 1)),
+                           cp.a);
+        }
+        const float C = klt_term(s0, v0, v1, v2, ga, k1, k2);
+        const float Cxp = klt_term(sp, v0, v1, v2, ga, k1, k2);
+        const float Cxm = klt_term(sm, v0, v1, v2, ga, k1, k2);
+        const float curv = (Cxp + Cxm) - 2.0f * C;
+        if (curv > 0.0f) {
+          const float gj = (w1 * (Cxm - Cxp)) * 0.5f;
+          J0 = J0 + gj * px;
+          J1 = J1 + gj * py;
+          J2 = J2 + gj * 1.0f;
+          const float h = w1 * curv;
+          H00 = H00 + h * (px * px);
+          H01 = H01 + h * (px * py);
+          H02 = H02 + h * (px * 1.0f);
+          H11 = H11 + h * (py * py);
+          H12 = H12 + h * (py * 1.0f);
+          H22 = H22 + h * (1.0f * 1.0f);
+        }
       }
     }
   }
@@ -245,9 +403,11 @@ __device__ float4 klt_pass(const ViewDev& v, int x, int y, float4 s) {
 
 // ---------------------------------------------------------------------
 // precompute: pmhstereo_bilateral_weight_to_alpha (pmhstereo.cl:1462-1498)
-// and pmhstereo_gradxy (pmhstereo.cl:1549-1602); the per-tap support
-// weights it sums are kept as planes for the cost and KLT loops
+// and pmhstereo_gradxy (pmhstereo.cl:1549-1602) into one texel record,
+// plus the gradient sign bits
 // ---------------------------------------------------------------------
+constexpr int kBX = 64, kBY = 4;  // simple per-pixel kernels
+
 __device__ __forceinline__ float4 fetch_rgba(const float* __restrict__ img,
                                              int C, int stride, int x, int y) {
   const float* r = img + (size_t)y * stride;
@@ -268,26 +428,22 @@ __device__ __forceinline__ float gray_at(const float* __restrict__ img, int W,
 
 __global__ void __launch_bounds__(256)
 k_prepare(const float* __restrict__ img, int W, int H, int C, int stride,
-          int R, float cw, size_t npix, float4* __restrict__ tex,
-          uint8_t* __restrict__ sgn, float* __restrict__ wts) {
+          int R, float cw, float4* __restrict__ tex,
+          uint8_t* __restrict__ sgn) {
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
   if (x >= W || y >= H) return;
   const size_t p = (size_t)y * W + x;
   const float4 c = fetch_rgba(img, C, stride, x, y);
-  const int T = 2 * R + 1;
   float wsum = 0.0f;
   for (int dy = -R; dy <= R; ++dy) {
     const int qy = y + dy;
+    if (qy < 0 || qy >= H) continue;
     for (int dx = -R; dx <= R; ++dx) {
       const int qx = x + dx;
-      float w = 0.0f;
-      if (qx >= 0 && qx < W && qy >= 0 && qy < H) {
-        const float4 q = fetch_rgba(img, C, stride, qx, qy);
-        w = support_weight(q.x - c.x, q.y - c.y, q.z - c.z, cw);
-        wsum += w;
-      }
-      wts[(size_t)((dy + R) * T + (dx + R)) * npix + p] = w;
+      if (qx < 0 || qx >= W) continue;
+      const float4 q = fetch_rgba(img, C, stride, qx, qy);
+      wsum += support_weight(q.x - c.x, q.y - c.y, q.z - c.z, cw);
     }
   }
   const float g = gray_at(img, W, H, C, stride, x, y);
@@ -301,34 +457,23 @@ k_prepare(const float* __restrict__ img, int W, int H, int C, int stride,
   sgn[p] = (uint8_t)signmask4(grad);
 }
 
-__device__ __forceinline__ ViewDev make_view(const Geometry& g, ViewBufs self,
-                                             ViewBufs other) {
-  ViewDev v;
-  v.tex1 = self.tex;
-  v.tex2 = other.tex;
-  v.sgn1 = self.sgn;
-  v.wts1 = self.wts;
-  v.W = g.W;
-  v.H = g.H;
-  v.R = g.R;
-  v.npix = g.npix;
-  return v;
-}
-
 // pmhstereo_init (pmhstereo.cl:755-773): RNG_init(p, 3), three draws
 __global__ void __launch_bounds__(256)
 k_init(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
        float4* __restrict__ states) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= g.W || y >= g.H) return;
-  const ViewDev v = make_view(g, self, other);
-  const size_t p = (size_t)y * g.W + x;
-  Mwc rng = mwc_jump((uint64_t)p * 3u);
-  const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
-  float4 s = state_init(u0, u1, u2, (float)x, (float)y, lr, dmax);
-  s.w = cost_eval(v, x, y, s, 1e10f);
-  states[p] = s;
+  extern __shared__ float4 smem[];
+  const Lane L = block_setup(g, self, other, smem);
+  float4 s = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (L.valid) {
+    Mwc rng = mwc_jump((uint64_t)L.p * 3u);
+    const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
+    s = state_init(u0, u1, u2, L.xf, L.yf, lr, dmax);
+  }
+  const float w = cost_eval<false>(L, s, 1e10f, true, nullptr);
+  if (L.valid) {
+    s.w = w;
+    states[L.p] = s;
+  }
 }
 
 // pmhstereo_disparity_to_normal (pmhstereo.cl:1604-1637)
@@ -354,24 +499,27 @@ __global__ void __launch_bounds__(256)
 k_init_prior(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
              const float* __restrict__ disp,
              const float4* __restrict__ normals, float4* __restrict__ states) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= g.W || y >= g.H) return;
-  const ViewDev v = make_view(g, self, other);
-  const size_t p = (size_t)y * g.W + x;
-  Mwc rng = mwc_jump((uint64_t)p * 3u);
-  const float d = disp[p];
-  float4 s;
-  if (d > 0.0f) {
-    const float4 nn = normals[p];
-    const float3 n = normalize3(make_float3(nn.x, nn.y, nn.z));
-    s = from_disparity_normal((float)x, (float)y, lr ? d : -d, n);
-  } else {
-    const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
-    s = state_init(u0, u1, u2, (float)x, (float)y, lr, dmax);
+  extern __shared__ float4 smem[];
+  const Lane L = block_setup(g, self, other, smem);
+  float4 s = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (L.valid) {
+    Mwc rng = mwc_jump((uint64_t)L.p * 3u);
+    const float d = disp[L.p];
+    if (d > 0.0f) {
+      const float4 nn = normals[L.p];
+      const float3 n = normalize3(make_float3(nn.x, nn.y, nn.z));
+      s = from_disparity_normal(L.xf, L.yf, lr ? d : -d, n);
+    } else {
+      const float u0 = mwc_next(&rng), u1 = mwc_next(&rng),
+                  u2 = mwc_next(&rng);
+      s = state_init(u0, u1, u2, L.xf, L.yf, lr, dmax);
+    }
   }
-  s.w = cost_eval(v, x, y, s, 1e10f);
-  states[p] = s;
+  const float w = cost_eval<false>(L, s, 1e10f, true, nullptr);
+  if (L.valid) {
+    s.w = w;
+    states[L.p] = s;
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -384,23 +532,29 @@ k_viewbuf_clear(size_t n, unsigned long long* __restrict__ vbuf) {
 // pmhstereo_propagate_view (pmhstereo.cl:880-1071), Jacobi schedule
 // ---------------------------------------------------------------------
 struct PropCtx {
-  float xf, yf, dmax, theta;
+  float dmax, theta;
   int lr;
   float4 smooth;
 };
 
 // candidate test with minimalCost (pmhstereo.cl:873-877) and the select
-// rule (pmhstereo.cl:976)
+// rule (pmhstereo.cl:976); called by the whole wavefront
 template <bool kCount>
-__device__ __forceinline__ void try_candidate(const ViewDev& v,
-                                              const PropCtx& c, int x, int y,
+__device__ __forceinline__ void try_candidate(const Lane& L, const PropCtx& c,
                                               float4& self, float4 cand,
-                                              uint32_t* ntaps) {
-  const float ds = smooth_dist(self, c.smooth, c.xf, c.yf, c.dmax, c.lr);
-  const float dc = smooth_dist(cand, c.smooth, c.xf, c.yf, c.dmax, c.lr);
-  const float mincost = (self.w + c.theta * ds) - c.theta * dc;
-  cand.w = cost_eval<kCount>(v, x, y, cand, mincost, ntaps);
-  if (cand.w + c.theta * dc <= self.w + c.theta * ds) self = cand;
+                                              bool active, uint32_t* ntaps) {
+  active = active && L.valid;
+  float ds = 0.0f, dc = 0.0f, mincost = 0.0f;
+  if (active) {
+    ds = smooth_dist(self, c.smooth, L.xf, L.yf, c.dmax, c.lr);
+    dc = smooth_dist(cand, c.smooth, L.xf, L.yf, c.dmax, c.lr);
+    mincost = (self.w + c.theta * ds) - c.theta * dc;
+  }
+  const float w = cost_eval<kCount>(L, cand, mincost, active, ntaps);
+  if (active) {
+    cand.w = w;
+    if (cand.w + c.theta * dc <= self.w + c.theta * ds) self = cand;
+  }
 }
 
 // keep the two smallest keys of a slot: the two nearest surfaces win the
@@ -445,35 +599,37 @@ __device__ __forceinline__ void scatter(float4 self, int x, int y, int W,
 template <bool kCount>
 __global__ void __launch_bounds__(256)
 k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= g.W || y >= g.H) return;
+  extern __shared__ float4 smem[];
+  const Lane L = block_setup(g, a.self, a.other, smem);
+  const int W = g.W, H = g.H;
   uint32_t ncost = 0;
   uint32_t* nc = kCount ? &ncost : nullptr;
-  const ViewDev v = make_view(g, a.self, a.other);
-  const int W = g.W, H = g.H;
-  const size_t p = (size_t)y * W + x;
   PropCtx c;
-  c.xf = (float)x;
-  c.yf = (float)y;
   c.dmax = a.dmax;
   c.theta = a.theta;
   c.lr = a.lr;
-  float4 sm = a.smooth ? a.smooth[p] : mk4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 sm = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 self = sm;
+  if (L.valid) {
+    if (a.smooth) sm = a.smooth[L.p];
+    self = a.old[L.p];
+  }
   sm.w = fminf(fmaxf(sm.w, 0.0f), 1.0f) * a.dmax;
   c.smooth = sm;
 
-  float4 self = a.old[p];
-  self.w = cost_eval<kCount>(v, x, y, self, 1e10f, nc);
-  Mwc rng = mwc_jump((uint64_t)(p + (size_t)a.iter) * kPropDraws);
+  {
+    const float w = cost_eval<kCount>(L, self, 1e10f, true, nc);
+    if (L.valid) self.w = w;
+  }
+  Mwc rng = mwc_jump((uint64_t)(L.p + (size_t)a.iter) * kPropDraws);
 
   // smooth proposal (pmhstereo.cl:968-976)
   {
     float4 cand = from_disparity_normal(
-        c.xf, c.yf, sm.w * (a.lr ? -1.0f : 1.0f),
+        L.xf, L.yf, sm.w * (a.lr ? -1.0f : 1.0f),
         normalize3(make_float3(sm.x, sm.y, sm.z)));
     if (!a.lr) cand = viewprop(cand);
-    try_candidate<kCount>(v, c, x, y, self, cand, nc);
+    try_candidate<kCount>(L, c, self, cand, true, nc);
   }
   // random neighbours (pmhstereo.cl:981-991), x offset drawn first
   for (int i = 0; i < kNumRndSample; ++i) {
@@ -481,49 +637,59 @@ k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
     const float uy = mwc_next(&rng);
     const int ox = (int)(((ux - 0.5f) * 2.0f) * kRndSampleRadius + 0.5f);
     const int oy = (int)(((uy - 0.5f) * 2.0f) * kRndSampleRadius + 0.5f);
-    const int nx = clampi(x + ox, 0, W - 1), ny = clampi(y + oy, 0, H - 1);
-    try_candidate<kCount>(v, c, x, y, self, a.old[(size_t)ny * W + nx],
-                          nc);
+    float4 cand = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (L.valid) {
+      const int nx = clampi(L.x + ox, 0, W - 1), ny = clampi(L.y + oy, 0, H - 1);
+      cand = a.old[(size_t)ny * W + nx];
+    }
+    try_candidate<kCount>(L, c, self, cand, true, nc);
   }
   // random plane (pmhstereo.cl:996-1002)
   {
     const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
-    try_candidate<kCount>(v, c, x, y, self,
-                          state_init(u0, u1, u2, c.xf, c.yf, a.lr, a.dmax),
-                          nc);
+    try_candidate<kCount>(L, c, self,
+                          state_init(u0, u1, u2, L.xf, L.yf, a.lr, a.dmax),
+                          true, nc);
   }
   // other view (pmhstereo.cl:1007-1016)
-  if (a.viewin) {
-    const unsigned long long slot = a.viewin[p];
-    const uint32_t keys[2] = {(uint32_t)(slot & 0xFFFFFFFFull),
-                              (uint32_t)(slot >> 32)};
+  {
+    uint32_t keys[2] = {kSlotEmpty, kSlotEmpty};
+    if (a.viewin && L.valid) {
+      const unsigned long long slot = a.viewin[L.p];
+      keys[0] = (uint32_t)(slot & 0xFFFFFFFFull);
+      keys[1] = (uint32_t)(slot >> 32);
+    }
     for (int i = 0; i < kViewSamples; ++i) {
-      if (keys[i] == kSlotEmpty) break;
-      const int xs = a.lr ? (int)keys[i] : W - 1 - (int)keys[i];
-      try_candidate<kCount>(v, c, x, y, self,
-                            viewprop(a.src_other[(size_t)y * W + xs]), nc);
+      const bool have = keys[0] != kSlotEmpty && keys[i] != kSlotEmpty;
+      float4 cand = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (have) {
+        const int xs = a.lr ? (int)keys[i] : W - 1 - (int)keys[i];
+        cand = viewprop(a.src_other[(size_t)L.y * W + xs]);
+      }
+      try_candidate<kCount>(L, c, self, cand, have, nc);
     }
   }
   // randomized refinement (pmhstereo.cl:1021-1028)
   {
     const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
     try_candidate<kCount>(
-        v, c, x, y, self,
-        state_refine(u0, u1, u2, self, c.xf, c.yf, a.dmax, a.lr), nc);
+        L, c, self, state_refine(u0, u1, u2, self, L.xf, L.yf, a.dmax, a.lr),
+        true, nc);
   }
   // KLT step (pmhstereo.cl:1034-1053)
   {
     float4 k = self;
-    for (int it = 0; it < kKltPasses; ++it) k = klt_pass(v, x, y, k);
-    try_candidate<kCount>(v, c, x, y, self, k, nc);
+    for (int it = 0; it < kKltPasses; ++it) k = klt_pass(L, k);
+    try_candidate<kCount>(L, c, self, k, true, nc);
   }
-  if (a.viewout) scatter(self, x, y, W, a.lr, a.viewout);
-  a.out[p] = self;
+  if (!L.valid) return;
+  if (a.viewout) scatter(self, L.x, L.y, W, a.lr, a.viewout);
+  a.out[L.p] = self;
   if (kCount) {
     // KLT taps: rows in the image times columns with 1 <= x+dx < W-1
     const int R = g.R;
-    const int rows = min(R, H - 1 - y) - max(-R, -y) + 1;
-    const int cols = min(R, W - 2 - x) - max(-R, 1 - x) + 1;
+    const int rows = min(R, H - 1 - L.y) - max(-R, -L.y) + 1;
+    const int cols = min(R, W - 2 - L.x) - max(-R, 1 - L.x) + 1;
     const unsigned long long nklt =
         cols > 0 ? (unsigned long long)rows * cols * kKltPasses : 0ull;
     atomicAdd(&counters[0], (unsigned long long)ncost);
@@ -545,26 +711,22 @@ k_scatter(int W, int H, const float4* __restrict__ states, int lr,
 __global__ void __launch_bounds__(256)
 k_cost_field(Geometry g, ViewBufs self, ViewBufs other,
              const float4* __restrict__ planes, float* __restrict__ cost) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= g.W || y >= g.H) return;
-  const ViewDev v = make_view(g, self, other);
-  const size_t p = (size_t)y * g.W + x;
-  cost[p] = cost_eval(v, x, y, planes[p], 1e10f);
+  extern __shared__ float4 smem[];
+  const Lane L = block_setup(g, self, other, smem);
+  const float4 s = L.valid ? planes[L.p] : mk4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float w = cost_eval<false>(L, s, 1e10f, true, nullptr);
+  if (L.valid) cost[L.p] = w;
 }
 
 __global__ void __launch_bounds__(256)
 k_klt_field(Geometry g, ViewBufs self, ViewBufs other,
             const float4* __restrict__ planes, int passes,
             float4* __restrict__ out) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= g.W || y >= g.H) return;
-  const ViewDev v = make_view(g, self, other);
-  const size_t p = (size_t)y * g.W + x;
-  float4 s = planes[p];
-  for (int i = 0; i < passes; ++i) s = klt_pass(v, x, y, s);
-  out[p] = s;
+  extern __shared__ float4 smem[];
+  const Lane L = block_setup(g, self, other, smem);
+  float4 s = L.valid ? planes[L.p] : mk4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int i = 0; i < passes; ++i) s = klt_pass(L, s);
+  if (L.valid) out[L.p] = s;
 }
 
 // pmhstereo_consistency (pmhstereo.cl:1095-1129)
@@ -715,119 +877,136 @@ k_math(const float* __restrict__ in, int n, float* __restrict__ out) {
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
-#define PM_LAUNCH(kernel, grid, block, st, ...)                 \
-  do {                                                          \
-    hipLaunchKernelGGL(kernel, grid, block, 0, st, __VA_ARGS__); \
-    return hipGetLastError();                                   \
+#define PM_LAUNCH(kernel, grid, block, lds, st, ...)                \
+  do {                                                              \
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__);  \
+    return hipGetLastError();                                       \
   } while (0)
 
 static const dim3 kBlock2D(kBX, kBY);
+static const dim3 kBlockTile(kTX, kTY);
+
+// dynamic LDS above 64 KiB has to be allowed per kernel
+template <typename K>
+static hipError_t allow_lds(K kernel, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
+
+#define PM_LAUNCH_TILE(kernel, g, st, ...)                              \
+  do {                                                                  \
+    const size_t lds_ = block_lds_bytes((g).R);                         \
+    hipError_t e_ = allow_lds(kernel, lds_);                            \
+    if (e_ != hipSuccess) return e_;                                    \
+    PM_LAUNCH(kernel, grid_for((g).W, (g).H), kBlockTile, lds_, st,     \
+              __VA_ARGS__);                                             \
+  } while (0)
 
 hipError_t launch_prepare(hipStream_t st, const Geometry& g, const float* img,
                           int channels, int stride, ViewBufs v) {
-  PM_LAUNCH(k_prepare, grid_for(g.W, g.H), kBlock2D, st, img, g.W, g.H,
-            channels, stride, g.R, g.color_weight, g.npix, v.tex, v.sgn,
-            v.wts);
+  const dim3 grid((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY);
+  PM_LAUNCH(k_prepare, grid, kBlock2D, 0, st, img, g.W, g.H, channels, stride,
+            g.R, g.color_weight, v.tex, v.sgn);
 }
 
 hipError_t launch_init(hipStream_t st, const Geometry& g, ViewBufs self,
                        ViewBufs other, float dmax, int lr, float4* states) {
-  PM_LAUNCH(k_init, grid_for(g.W, g.H), kBlock2D, st, g, self, other, dmax,
-            lr, states);
+  PM_LAUNCH_TILE(k_init, g, st, g, self, other, dmax, lr, states);
 }
 
 hipError_t launch_disparity_to_normal(hipStream_t st, const Geometry& g,
                                       const float* disp, float4* normals) {
-  PM_LAUNCH(k_disparity_to_normal, grid_for(g.W, g.H), kBlock2D, st, g.W, g.H,
-            disp, normals);
+  PM_LAUNCH(k_disparity_to_normal,
+            dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY), kBlock2D, 0,
+            st, g.W, g.H, disp, normals);
 }
 
 hipError_t launch_init_prior(hipStream_t st, const Geometry& g, ViewBufs self,
                              ViewBufs other, float dmax, int lr,
                              const float* disp, const float4* normals,
                              float4* states) {
-  PM_LAUNCH(k_init_prior, grid_for(g.W, g.H), kBlock2D, st, g, self, other,
-            dmax, lr, disp, normals, states);
+  PM_LAUNCH_TILE(k_init_prior, g, st, g, self, other, dmax, lr, disp, normals,
+                 states);
 }
 
 hipError_t launch_viewbuf_clear(hipStream_t st, const Geometry& g,
                                 unsigned long long* vbuf) {
   const unsigned blocks = (unsigned)((g.npix + 255) / 256);
-  PM_LAUNCH(k_viewbuf_clear, dim3(blocks), dim3(256), st, g.npix, vbuf);
+  PM_LAUNCH(k_viewbuf_clear, dim3(blocks), dim3(256), 0, st, g.npix, vbuf);
 }
 
 hipError_t launch_propagate(hipStream_t st, const Geometry& g,
                             const PropagateArgs& a,
                             unsigned long long* counters) {
-  if (counters)
-    PM_LAUNCH(k_propagate<true>, grid_for(g.W, g.H), kBlock2D, st, g, a,
-              counters);
-  PM_LAUNCH(k_propagate<false>, grid_for(g.W, g.H), kBlock2D, st, g, a,
-            counters);
+  if (counters) PM_LAUNCH_TILE(k_propagate<true>, g, st, g, a, counters);
+  PM_LAUNCH_TILE(k_propagate<false>, g, st, g, a, counters);
 }
 
 hipError_t launch_scatter(hipStream_t st, const Geometry& g,
                           const float4* states, int lr,
                           unsigned long long* viewout) {
-  PM_LAUNCH(k_scatter, grid_for(g.W, g.H), kBlock2D, st, g.W, g.H, states, lr,
-            viewout);
+  PM_LAUNCH(k_scatter, dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY),
+            kBlock2D, 0, st, g.W, g.H, states, lr, viewout);
 }
 
 hipError_t launch_cost_field(hipStream_t st, const Geometry& g, ViewBufs self,
                              ViewBufs other, const float4* planes,
                              float* cost) {
-  PM_LAUNCH(k_cost_field, grid_for(g.W, g.H), kBlock2D, st, g, self, other,
-            planes, cost);
+  PM_LAUNCH_TILE(k_cost_field, g, st, g, self, other, planes, cost);
 }
 
 hipError_t launch_klt_field(hipStream_t st, const Geometry& g, ViewBufs self,
                             ViewBufs other, const float4* planes, int passes,
                             float4* out) {
-  PM_LAUNCH(k_klt_field, grid_for(g.W, g.H), kBlock2D, st, g, self, other,
-            planes, passes, out);
+  PM_LAUNCH_TILE(k_klt_field, g, st, g, self, other, planes, passes, out);
 }
 
 hipError_t launch_consistency(hipStream_t st, const Geometry& g,
                               const float4* left, const float4* right,
                               float maxdispdiff, float maxdeg, float4* out) {
-  PM_LAUNCH(k_consistency, grid_for(g.W, g.H), kBlock2D, st, g.W, g.H, left,
-            right, maxdispdiff, maxdeg, out);
+  PM_LAUNCH(k_consistency, dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY),
+            kBlock2D, 0, st, g.W, g.H, left, right, maxdispdiff, maxdeg, out);
 }
 
 hipError_t launch_to_disparity(hipStream_t st, const Geometry& g,
                                const float4* states, float scale, int lr,
                                float* disp) {
-  PM_LAUNCH(k_to_disparity, grid_for(g.W, g.H), kBlock2D, st, g.W, g.H,
-            states, scale, lr, disp);
+  PM_LAUNCH(k_to_disparity,
+            dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY), kBlock2D, 0,
+            st, g.W, g.H, states, scale, lr, disp);
 }
 
 hipError_t launch_fill_depthmap(hipStream_t st, const Geometry& g,
                                 const float4* states, float scale,
                                 float* disp) {
-  PM_LAUNCH(k_fill_depthmap, grid_for(g.W, g.H), kBlock2D, st, g.W, g.H,
-            states, scale, disp);
+  PM_LAUNCH(k_fill_depthmap,
+            dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY), kBlock2D, 0,
+            st, g.W, g.H, states, scale, disp);
 }
 
 hipError_t launch_fill_normalmap(hipStream_t st, const Geometry& g,
                                  const float4* states, float4* normals) {
-  PM_LAUNCH(k_fill_normalmap, grid_for(g.W, g.H), kBlock2D, st, g.W, g.H,
-            states, normals);
+  PM_LAUNCH(k_fill_normalmap,
+            dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY), kBlock2D, 0,
+            st, g.W, g.H, states, normals);
 }
 
 hipError_t launch_draws(hipStream_t st, uint64_t k, int n, float* out) {
   const int threads = (n + 15) / 16;
-  PM_LAUNCH(k_draws, dim3((unsigned)((threads + 255) / 256)), dim3(256), st,
-            k, n, out);
+  PM_LAUNCH(k_draws, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+            st, k, n, out);
 }
 
 hipError_t launch_copy_probe(hipStream_t st, const float4* src, float4* dst,
                              size_t n) {
-  PM_LAUNCH(k_copy_probe, dim3(4096), dim3(256), st, src, dst, n);
+  PM_LAUNCH(k_copy_probe, dim3(4096), dim3(256), 0, st, src, dst, n);
 }
 
 hipError_t launch_math(hipStream_t st, const float* in, int n, float* out) {
-  PM_LAUNCH(k_math, dim3((unsigned)((n + 255) / 256)), dim3(256), st, in, n,
-            out);
+  PM_LAUNCH(k_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in,
+            n, out);
 }
 
 }  // namespace pm
