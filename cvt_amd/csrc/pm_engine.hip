@@ -83,6 +83,13 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// sign bits of a gradient as one byte per component: byte k is 0x80 when
+// component k has its sign bit set (the tile's .w holds the own view's)
+__device__ __forceinline__ uint32_t expand_signs(uint32_t m) {
+  return ((m & 1u) << 7) | ((m & 2u) << 14) | ((m & 4u) << 21) |
+         ((m & 8u) << 28);
+}
+
 // Work-group (bx, by) of the image for the hardware work-group id.
 // Consecutive work-groups are dispatched round-robin over the XCDs; this
 // numbering gives XCD k the k-th contiguous eighth of the blocks.
@@ -130,7 +137,7 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
     if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
       const size_t q = (size_t)gy * W + gx;
       const float4 a = self.tex[2 * q];
-      v = mk4(a.x, a.y, a.z, __uint_as_float((uint32_t)self.sgn[q]));
+      v = mk4(a.x, a.y, a.z, __uint_as_float(expand_signs(self.sgn[q])));
     }
     smem[i] = v;
   }
@@ -225,9 +232,15 @@ __device__ __forceinline__ Sample sample_inside(
   return lerp_texels(texel(srow, i0), texel(srow, i1), u - fl);
 }
 
-// gradient sign mismatches between sign bits m1 and the signs of g
-__device__ __forceinline__ int sign_mismatches(uint32_t m1, float4 g) {
-  return __popc(m1 ^ signmask4(g));
+// gradient sign mismatches between expanded sign bytes m1x and the signs
+// of g: gather the top byte of each component (v_perm_b32), xor, keep the
+// sign bits, count
+__device__ __forceinline__ int sign_mismatches(uint32_t m1x, float4 g) {
+  const uint32_t lo = __builtin_amdgcn_perm(__float_as_uint(g.y),
+                                            __float_as_uint(g.x), 0x0c0c0703u);
+  const uint32_t hi = __builtin_amdgcn_perm(__float_as_uint(g.w),
+                                            __float_as_uint(g.z), 0x07030c0cu);
+  return __popc(((lo | hi) ^ m1x) & 0x80808080u);
 }
 
 // ---------------------------------------------------------------------
@@ -336,29 +349,35 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
         const float w1 = support_weight(v0, v1, v2, L.cw);
         const float k1 = (w1 * 0.125f) * 0.33f, k2 = w1 * 0.875f;
         const float xq = fmaf(s.x, px, rowc);
-        const Coord cm = coord_of(L, xq - 1.0f);
-        const Coord c0 = coord_of(L, xq);
-        const Coord cp = coord_of(L, xq + 1.0f);
+        const float xp1 = xq + 1.0f;
         Sample sm, s0, sp;
-        if (__all(c0.f == cm.f + 1 && cp.f == cm.f + 2)) {
-          // consecutive floors: the three samples share four texels
-          const Texel t0 = texel(srow, (uint32_t)clampi(cm.f, 0, W - 1));
-          const Texel t1 = texel(srow, (uint32_t)clampi(cm.f + 1, 0, W - 1));
-          const Texel t2 = texel(srow, (uint32_t)clampi(cm.f + 2, 0, W - 1));
-          const Texel t3 = texel(srow, (uint32_t)clampi(cm.f + 3, 0, W - 1));
-          sm = lerp_texels(t0, t1, cm.a);
-          s0 = lerp_texels(t1, t2, c0.a);
-          sp = lerp_texels(t2, t3, cp.a);
+        // For 1 <= x' and x'+1 < W-1 no sample needs clamping; x'-1 is then
+        // exact, so it shares the floor and fraction of x'; x'+1 (rounded)
+        // has its own fraction and, unless it rounded up to the next
+        // integer, floor(x')+1: the three samples read four consecutive
+        // texels.
+        const float fl = floorf(xq);
+        const float flp = floorf(xp1);
+        if (__all(xq >= 1.0f && xp1 < L.Wf - 1.0f && flp == fl + 1.0f)) {
+          const float a = xq - fl;
+          const float ap = xp1 - flp;
+          const float4* __restrict__ t = srow + 2 * ((uint32_t)(int)fl - 1u);
+          Texel t0, t1, t2, t3;
+          t0.A = t[0];
+          t0.G = t[1];
+          t1.A = t[2];
+          t1.G = t[3];
+          t2.A = t[4];
+          t2.G = t[5];
+          t3.A = t[6];
+          t3.G = t[7];
+          sm = lerp_texels(t0, t1, a);
+          s0 = lerp_texels(t1, t2, a);
+          sp = lerp_texels(t2, t3, ap);
         } else {
-          sm = lerp_texels(texel(srow, (uint32_t)clampi(cm.f, 0, W - 1)),
-                           texel(srow, (uint32_t)clampi(cm.f + 1, 0, W - 1)),
-                           cm.a);
-          s0 = lerp_texels(texel(srow, (uint32_t)clampi(c0.f, 0, W - 1)),
-                           texel(srow, (uint32_t)clampi(c0.f + 1, 0, W - 1)),
-                           c0.a);
-          sp = lerp_texels(texel(srow, (uint32_t)clampi(cp.f, 0, W - 1)),
-                           texel(srow, (uint32_t)clampi(cp.f + 1, 0, W - 1)),
-                           cp.a);
+          sm = sample(L, srow, xq - 1.0f);
+          s0 = sample(L, srow, xq);
+          sp = sample(L, srow, xp1);
         }
         const float C = klt_term(s0, v0, v1, v2, ga, k1, k2);
         const float Cxp = klt_term(sp, v0, v1, v2, ga, k1, k2);
