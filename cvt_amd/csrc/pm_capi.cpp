@@ -124,6 +124,8 @@ int check_params(pm_ctx* c, const pm_params* p) {
     return fail(c, PM_ERR_ARG, "color_weight must be finite and >= 0");
   if (!(p->theta >= 0.0f) || !std::isfinite(p->theta))
     return fail(c, PM_ERR_ARG, "theta must be finite and >= 0");
+  if (p->device < 0)
+    return fail(c, PM_ERR_ARG, "device ordinal %d is negative", p->device);
   return PM_OK;
 }
 
