@@ -26,6 +26,7 @@ METRIC = ("720p stereo pairs/s at matched bad-1.0px; learned-init vs plain "
           "PatchMatch")
 W, H, CELLS, DMAX, RADIUS, ITERS, SEED = 1280, 720, 24, 128.0, 4, 8, 0
 MAXDISP, MAXDEG = 1.0, 180.0
+SAMPLE_ITERS = 3     # iterations the CPU baseline times (of ITERS)
 # algorithmic FP32 work per valid tap (FMA = 2 FLOP):
 #  cost tap  = transform 4 + colour lerp 10 + gradient lerp 13 + colour term
 #              13 + sign term 6 + accumulate/exit 4                 = 50
@@ -67,8 +68,9 @@ def profiled_traffic():
 
 def cpu_baseline(L, R, gt):
     """The oracle (C, OpenMP on all host cores) on a bounded sample: the
-    full setup, one of the eight iterations and the post-processing of the
-    same pair; per-pair time = setup + 8 x iteration + post."""
+    full setup, SAMPLE_ITERS of the eight iterations and the
+    post-processing of the same pair; per-pair time = setup + 8 x the mean
+    iteration + post."""
     import numpy as np
     from oracle import oracle as O
     t0 = time.perf_counter()
@@ -77,19 +79,22 @@ def cpu_baseline(L, R, gt):
     sr = O.init(tr, tl, RADIUS, DMAX, 0)
     t1 = time.perf_counter()
     vb2 = O.viewbuf_empty(W, H)
-    sl, vb1 = O.propagate(tl, tr, RADIUS, DMAX, 1, 0, sl, vb2, sr)
-    sr, vb2 = O.propagate(tr, tl, RADIUS, DMAX, 0, 0, sr, vb1, sl)
+    for it in range(SAMPLE_ITERS):
+        sl, vb1 = O.propagate(tl, tr, RADIUS, DMAX, 1, it, sl, vb2, sr)
+        sr, vb2 = O.propagate(tr, tl, RADIUS, DMAX, 0, it, sr, vb1, sl)
     t2 = time.perf_counter()
     d = O.to_disparity(O.consistency(sl, sr, MAXDISP, MAXDEG))
     t3 = time.perf_counter()
-    per_pair = (t1 - t0) + ITERS * (t2 - t1) + (t3 - t2)
+    per_iter = (t2 - t1) / SAMPLE_ITERS
+    per_pair = (t1 - t0) + ITERS * per_iter + (t3 - t2)
     del d, np
     return {"value": 1.0 / per_pair, "unit": "pairs/s",
             "cores": O.num_threads(), "kind": "port",
-            "sample": "1280x720 pair: setup + 1 of %d iterations + LR check "
-                      "timed (%.1f s); per pair = setup %.2f s + %d x "
+            "sample": "1280x720 pair: setup + %d of %d iterations + LR "
+                      "check timed (%.1f s); per pair = setup %.2f s + %d x "
                       "iteration %.2f s + post %.3f s" %
-                      (ITERS, t3 - t0, t1 - t0, ITERS, t2 - t1, t3 - t2),
+                      (SAMPLE_ITERS, ITERS, t3 - t0, t1 - t0, ITERS, per_iter,
+                       t3 - t2),
             "ms_per_pair": per_pair * 1e3}
 
 
@@ -99,6 +104,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--io-steps", type=int, default=2,
+                    help="pairs run from and to host memory after the timed "
+                         "region (PCIe-inclusive rate)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the engine's plain copy kernel over 512 "
                          "MiB (calibration run for the HBM byte counters)")
@@ -181,6 +189,15 @@ def main():
         iter_ms = [ev[k][i][0].elapsed_time(ev[k][i][1])
                    for k in range(args.steps) for i in range(ITERS)]
         prop_ms = float(np.mean(iter_ms)) / 2.0
+        # the same pair from and to host memory (pm_set_images /
+        # pm_disparity): the PCIe-inclusive rate, reported beside `value`
+        t2 = time.perf_counter()
+        for k in range(args.io_steps):
+            ctx.set_images(L, R)
+            ctx.setup()
+            ctx.iterate(ITERS)
+            ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP, MAXDEG)
+        t3 = time.perf_counter()
         # untimed counting pass: identical inputs, identical work
         ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
         ctx.setup()
@@ -215,6 +232,11 @@ def main():
                    "radius": RADIUS, "iterations": ITERS,
                    "disparity_max": DMAX, "seed": SEED},
         "quality": {"bad_1px": bad1},
+        "host_io": ({"value": args.io_steps / (t3 - t2), "unit": "pairs/s",
+                     "ms_per_pair": (t3 - t2) / args.io_steps * 1e3,
+                     "note": "pair uploaded from and disparity returned to "
+                             "host memory per step"}
+                    if args.io_steps > 0 else None),
         "roofline": {"bound": "valu", "kernel": "k_propagate",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
