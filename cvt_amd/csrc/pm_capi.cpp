@@ -141,7 +141,9 @@ int alloc_pair(pm_ctx* c, int W, int H, int channels) {
   int rc;
   for (int v = 0; v < 2; ++v) {
     if ((rc = ensure(c, c->img[v], n * channels * sizeof(float)))) return rc;
-    if ((rc = ensure(c, c->tex[v], n * 2 * sizeof(float4)))) return rc;
+    if ((rc = ensure(c, c->tex[v], (size_t)pm::tex_stride(W) * H * 2 *
+                                       sizeof(float4))))
+      return rc;
     if ((rc = ensure(c, c->sgn[v], n))) return rc;
     for (int k = 0; k < 2; ++k)
       if ((rc = ensure(c, c->state[v][k], n * sizeof(float4)))) return rc;
@@ -155,6 +157,7 @@ int alloc_pair(pm_ctx* c, int W, int H, int channels) {
   c->geo.W = W;
   c->geo.H = H;
   c->geo.R = c->prm.radius;
+  c->geo.TS = pm::tex_stride(W);
   c->geo.npix = n;
   c->geo.color_weight = c->prm.color_weight;
   c->channels = channels;
@@ -573,7 +576,15 @@ int pm_get_texels(pm_ctx* c, int view, float* out) {
       (rc = use_device(c)))
     return rc;
   if (!out) return fail(c, PM_ERR_ARG, "out is NULL");
-  return to_host(c, out, c->tex[view].ptr, c->geo.npix * 2 * sizeof(float4));
+  // the W texels of each row, without the edge padding
+  const size_t rec = 2 * sizeof(float4);
+  const char* src =
+      static_cast<const char*>(c->tex[view].ptr) + pm::kTexPadL * rec;
+  PM_HIP(c, hipMemcpy2DAsync(out, c->geo.W * rec, src, c->geo.TS * rec,
+                             c->geo.W * rec, c->geo.H, hipMemcpyDeviceToHost,
+                             c->stream));
+  PM_HIP(c, hipStreamSynchronize(c->stream));
+  return PM_OK;
 }
 
 int pm_draws(pm_ctx* c, uint64_t k, int n, float* out) {

@@ -10,15 +10,28 @@
 namespace pm {
 
 // Device buffers of one view.  A texel record is two float4:
-// tex[2p] = (r, g, b, bilw), tex[2p+1] = (gx, gy, gxy, gyx);
-// sgn[p] = sign bits of the gradient (bit k = signbit of component k).
+// (r, g, b, bilw) then (gx, gy, gxy, gyx).  Texel rows are padded with
+// copies of their edge texels, kTexPadL on the left and kTexPadR on the
+// right, so that every linear sample the kernels form (coordinates
+// clamped to [-1, W]) reads existing records without clamping its
+// indices: texel (x, y) is record y * tex_stride(W) + kTexPadL + x.
+// sgn[y * W + x] = sign bits of the gradient (bit k = signbit of
+// component k).
 struct ViewBufs {
   float4* tex = nullptr;
   uint8_t* sgn = nullptr;
 };
 
+constexpr int kTexPadL = 2;
+constexpr int kTexPadR = 3;
+
+__host__ __device__ inline int tex_stride(int W) {
+  return W + kTexPadL + kTexPadR;
+}
+
 struct Geometry {
   int W = 0, H = 0, R = 0;
+  int TS = 0;  // texel row stride in records, tex_stride(W)
   size_t npix = 0;
   float color_weight = 12.0f;
 };

@@ -83,11 +83,11 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
-// sign bits of a gradient as one byte per component: byte k is 0x80 when
+// sign bits of a gradient as one byte per component: byte k is 0xff when
 // component k has its sign bit set (the tile's .w holds the own view's)
 __device__ __forceinline__ uint32_t expand_signs(uint32_t m) {
-  return ((m & 1u) << 7) | ((m & 2u) << 14) | ((m & 4u) << 21) |
-         ((m & 8u) << 28);
+  return ((m & 1u) ? 0x000000ffu : 0u) | ((m & 2u) ? 0x0000ff00u : 0u) |
+         ((m & 4u) ? 0x00ff0000u : 0u) | ((m & 8u) ? 0xff000000u : 0u);
 }
 
 // Work-group (bx, by) of the image for the hardware work-group id.
@@ -117,7 +117,8 @@ struct Lane {
   const float4* trow0;  // LDS tile at this lane: tap (dx, dy) is
                         // trow0[(dy + R) * TW + dx + R]
   int TW;
-  const float4* __restrict__ tex2;  // other view's texel records (global)
+  int TS;               // texel row stride (records, with the edge pads)
+  const float4* __restrict__ tex2;  // other view's texel (0, 0), global
 };
 
 // Block prologue shared by the cost-evaluating kernels: stages the own-view
@@ -135,9 +136,9 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
     const int gx = x0 - R + i % TW, gy = y0 - R + i / TW;
     float4 v = mk4(0.0f, 0.0f, 0.0f, 0.0f);
     if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-      const size_t q = (size_t)gy * W + gx;
-      const float4 a = self.tex[2 * q];
-      v = mk4(a.x, a.y, a.z, __uint_as_float(expand_signs(self.sgn[q])));
+      const float4 a = self.tex[2 * ((size_t)gy * g.TS + kTexPadL + gx)];
+      v = mk4(a.x, a.y, a.z,
+              __uint_as_float(expand_signs(self.sgn[(size_t)gy * W + gx])));
     }
     smem[i] = v;
   }
@@ -157,10 +158,13 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
   L.yf = (float)L.y;
   L.p = L.valid ? (size_t)L.y * W + L.x : 0;
   L.TW = TW;
+  L.TS = g.TS;
   L.trow0 = smem + ly * TW + threadIdx.x;
-  L.tex2 = other.tex;
+  L.tex2 = other.tex + 2 * kTexPadL;
   L.tc = L.trow0[R * TW + R];
-  L.bilw = L.valid ? self.tex[2 * L.p].w : 1.0f;
+  L.bilw = L.valid
+               ? self.tex[2 * ((size_t)L.y * g.TS + kTexPadL + L.x)].w
+               : 1.0f;
   return L;
 }
 
@@ -222,25 +226,25 @@ __device__ __forceinline__ Sample sample(const Lane& L,
   return lerp_texels(texel(srow, i0), texel(srow, i1), c.a);
 }
 
-// sample() for 0 <= u < W, where the coordinate and the first index need
-// no clamping
-__device__ __forceinline__ Sample sample_inside(
-    const Lane& L, const float4* __restrict__ srow, float u) {
+// sample() for 0 <= u < W: no coordinate clamp, and the second texel of
+// u in [W-1, W) is the right edge pad, a copy of texel W-1
+__device__ __forceinline__ Sample sample_inside(const float4* __restrict__ srow,
+                                                float u) {
   const float fl = floorf(u);
   const uint32_t i0 = (uint32_t)(int)fl;
-  const uint32_t i1 = min(i0 + 1u, (uint32_t)(L.W - 1));
-  return lerp_texels(texel(srow, i0), texel(srow, i1), u - fl);
+  return lerp_texels(texel(srow, i0), texel(srow, i0 + 1u), u - fl);
 }
 
-// gradient sign mismatches between expanded sign bytes m1x and the signs
-// of g: gather the top byte of each component (v_perm_b32), xor, keep the
-// sign bits, count
-__device__ __forceinline__ int sign_mismatches(uint32_t m1x, float4 g) {
+// 8 x the gradient sign mismatches between expanded sign bytes m1x and
+// the signs of g.  v_perm_b32 selectors 9 and 11 replicate the sign bit of
+// the low and the high source operand into a whole byte, so the signs of
+// g arrive in the layout of m1x; xor and count.
+__device__ __forceinline__ int sign_mismatches8(uint32_t m1x, float4 g) {
   const uint32_t lo = __builtin_amdgcn_perm(__float_as_uint(g.y),
-                                            __float_as_uint(g.x), 0x0c0c0703u);
+                                            __float_as_uint(g.x), 0x0c0c0b09u);
   const uint32_t hi = __builtin_amdgcn_perm(__float_as_uint(g.w),
-                                            __float_as_uint(g.z), 0x07030c0cu);
-  return __popc(((lo | hi) ^ m1x) & 0x80808080u);
+                                            __float_as_uint(g.z), 0x0b090c0cu);
+  return __popc((lo | hi) ^ m1x);
 }
 
 // ---------------------------------------------------------------------
@@ -274,7 +278,7 @@ __device__ float cost_eval(const Lane& L, float4 s, float mincost,
     if (qy < 0 || qy >= L.H) continue;
     if (!live) continue;
     const float rowc = fmaf(s.y, (float)qy, s.z);
-    const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * W;
+    const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * L.TS;
     const float4* trow = L.trow0 + (dy + R) * L.TW + R;
     for (int dx = dx0; dx <= dx1; ++dx) {
       const float xq = fmaf(s.x, (float)(L.x + dx), rowc);
@@ -283,14 +287,16 @@ __device__ float cost_eval(const Lane& L, float4 s, float mincost,
       const uint32_t m1 = __float_as_uint(t1.w);
       const float w = support_weight(t1.x - L.tc.x, t1.y - L.tc.y,
                                      t1.z - L.tc.z, L.cw);
-      const Sample s2 = sample_inside(L, srow, xq);
+      const Sample s2 = sample_inside(srow, xq);
       const float c0 = fminf(fabsf(s2.r - t1.x), 0.05f) * 20.0f;
       const float c1 = fminf(fabsf(s2.g - t1.y), 0.05f) * 20.0f;
       const float c2 = fminf(fabsf(s2.b - t1.z), 0.05f) * 20.0f;
       const float cad = 0.33f * ((c0 + c1) + c2);
       // cost = 0.125 cad, then cost += 0.875 * (0.25 n): both products
-      // are exact, so one fma with 0.21875 n rounds the same sum
-      const float cost = fmaf(0.21875f, (float)sign_mismatches(m1, s2.grad),
+      // are exact, so one fma with 0.21875 n = (7/256) (8 n) rounds the
+      // same sum
+      const float cost = fmaf(0.02734375f,
+                              (float)sign_mismatches8(m1, s2.grad),
                               0.125f * cad);
       census = fmaf(w, cost, census);
       wsum += w;
@@ -319,8 +325,8 @@ __device__ __forceinline__ float klt_term(const Sample& s, float v0, float v1,
   const float c2 = fminf(fabsf(s.b - v2), 0.05f) * 20.0f;
   const float C = k1 * ((c0 + c1) + c2);
   // C += w1 * (1 - alpha) * 0.25 * n, 0.25 a double literal
-  // (pmhstereo.cl:370): exact product, one rounding
-  return fmaf(k2, 0.25f * (float)sign_mismatches(ga, s.grad), C);
+  // (pmhstereo.cl:370): 0.25 n = 0.03125 (8 n) is exact, one rounding
+  return fmaf(k2, 0.03125f * (float)sign_mismatches8(ga, s.grad), C);
 }
 
 // patch_census_klt2 (pmhstereo.cl:326-410) + mat3f_lusolve
@@ -338,7 +344,7 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
       if (qy < 0 || qy >= L.H) continue;
       const float py = (float)qy;
       const float rowc = fmaf(s.y, py, s.z);
-      const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * W;
+      const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * L.TS;
       const float4* trow = L.trow0 + (dy + R) * L.TW + R;
       for (int dx = dx0; dx <= dx1; ++dx) {
         const float px = (float)(L.x + dx);
@@ -349,19 +355,25 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
         const float w1 = support_weight(v0, v1, v2, L.cw);
         const float k1 = (w1 * 0.125f) * 0.33f, k2 = w1 * 0.875f;
         const float xq = fmaf(s.x, px, rowc);
-        const float xp1 = xq + 1.0f;
         Sample sm, s0, sp;
-        // For 1 <= x' and x'+1 < W-1 no sample needs clamping; x'-1 is then
-        // exact, so it shares the floor and fraction of x'; x'+1 (rounded)
-        // has its own fraction and, unless it rounded up to the next
-        // integer, floor(x')+1: the three samples read four consecutive
-        // texels.
-        const float fl = floorf(xq);
+        // The samples at x'-1, x' and x'+1 read the four texels from
+        // floor(xc)-1 on, xc = x' clamped to [-1, W]:
+        //  - x'-1 is exact for x' >= 0.5 and shares the floor and fraction
+        //    of x'; below that it and its neighbour both read texel 0 (the
+        //    left pads), whatever the fraction;
+        //  - past either edge every texel involved is a copy of the edge
+        //    texel, which is what the clamped sampler returns;
+        //  - x'+1 is rounded and has its own fraction; its floor is
+        //    floor(xc)+1 unless the sum rounded up to the next integer,
+        //    which takes the general path.
+        const float xc = fmaxf(fminf(xq, L.Wf), -1.0f);
+        const float xp1 = xc + 1.0f;
+        const float fl = floorf(xc);
         const float flp = floorf(xp1);
-        if (__all(xq >= 1.0f && xp1 < L.Wf - 1.0f && flp == fl + 1.0f)) {
-          const float a = xq - fl;
+        if (__all(flp == fl + 1.0f)) {
+          const float a = xc - fl;
           const float ap = xp1 - flp;
-          const float4* __restrict__ t = srow + 2 * ((uint32_t)(int)fl - 1u);
+          const float4* __restrict__ t = srow + 2 * ((int)fl - 1);
           Texel t0, t1, t2, t3;
           t0.A = t[0];
           t0.G = t[1];
@@ -471,8 +483,21 @@ k_prepare(const float* __restrict__ img, int W, int H, int C, int stride,
   const float gyp = gray_at(img, W, H, C, stride, x, y + 1);
   const float gym = gray_at(img, W, H, C, stride, x, y - 1);
   const float4 grad = mk4(gxp - g, gyp - g, gxp - gxm, gyp - gym);
-  tex[2 * p] = mk4(c.x, c.y, c.z, wsum);
-  tex[2 * p + 1] = grad;
+  const float4 col = mk4(c.x, c.y, c.z, wsum);
+  float4* row = tex + 2 * ((size_t)y * tex_stride(W) + kTexPadL);
+  row[2 * x] = col;
+  row[2 * x + 1] = grad;
+  // edge pads: copies of the first and last texel of the row
+  if (x == 0)
+    for (int i = 1; i <= kTexPadL; ++i) {
+      row[-2 * i] = col;
+      row[-2 * i + 1] = grad;
+    }
+  if (x == W - 1)
+    for (int i = 1; i <= kTexPadR; ++i) {
+      row[2 * (x + i)] = col;
+      row[2 * (x + i) + 1] = grad;
+    }
   sgn[p] = (uint8_t)signmask4(grad);
 }
 
