@@ -211,7 +211,7 @@ def main():
     achieved = flop / (prop_ms * 1e-3) / 1e12
     elapsed = t1 - t0
     value = args.steps / elapsed
-    bad1 = synth.bad_pixel_rate(disp, gt, 1.0)
+    bad1 = cvt_amd.disparity.bad_pixel_rate(disp, gt, 1.0)
     result = {
         "metric": METRIC,
         "value": value,

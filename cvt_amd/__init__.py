@@ -2,8 +2,10 @@
 
 Drop-in for the setup/iterate path of cvt::PMHuberStereo: hand-written HIP
 kernels for gfx950 behind the C ABI of include/pm_stereo.h, a C++ class
-(include/cvt_amd/PMHuberStereo.h) and this Python interface.
+(include/cvt_amd/PMHuberStereo.h) and this Python interface; PFM disparity
+maps and cvt's disparity error metrics in ``pfm`` and ``disparity``.
 """
+from . import disparity, pfm  # noqa: F401
 from .stereo import (DISPARITY_PP_LRCHECK, DISPARITY_PP_LRCHECK_FILL,  # noqa
                      DISPARITY_PP_NONE, Context, PMError, PMHuberStereo)
 

@@ -10,7 +10,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 SO_PATH = os.path.join(HERE, "libcvt_amd_pm.so")
 
-PM_OK, PM_ERR_ARG, PM_ERR_STATE, PM_ERR_HIP, PM_ERR_NOMEM = range(5)
+PM_OK, PM_ERR_ARG, PM_ERR_STATE, PM_ERR_HIP, PM_ERR_NOMEM, PM_ERR_IO = range(6)
 PM_PP_NONE, PM_PP_LRCHECK, PM_PP_LRCHECK_FILL = range(3)
 
 _p = ctypes.c_void_p
@@ -28,7 +28,8 @@ class PmParams(ctypes.Structure):
                 ("stream", ctypes.c_void_p)]
 
 
-# name -> (restype, argtypes); every symbol include/pm_stereo.h declares
+# name -> (restype, argtypes); every symbol include/pm_stereo.h and
+# include/pm_io.h declare
 SIGNATURES = {
     "pm_params_default": (None, [ctypes.POINTER(PmParams)]),
     "pm_create": (_i, [ctypes.POINTER(PmParams), ctypes.POINTER(_p)]),
@@ -53,6 +54,14 @@ SIGNATURES = {
     "pm_iterate_counted": (_i, [_p, _i, _p, _p]),
     "pm_copy_probe": (_i, [_p, _p, _p, ctypes.c_size_t]),
     "pm_math_selftest": (_i, [_p, _p, _i, _p]),
+    # include/pm_io.h
+    "pm_io_last_error": (ctypes.c_char_p, []),
+    "pm_pfm_info": (_i, [ctypes.c_char_p, _p, _p, _p]),
+    "pm_pfm_read": (_i, [ctypes.c_char_p, _p, _i, _i, _i, _i]),
+    "pm_pfm_write": (_i, [ctypes.c_char_p, _p, _i, _i, _i, _i]),
+    "pm_eval_error": (_i, [_p, _i, _p, _i, _i, _i, _f, _f, _p, _p, _p]),
+    "pm_error_histogram": (_i, [_p, _i, _p, _i, _i, _i, _i, _p, _i, _p]),
+    "pm_default_histogram": (_i, [_i, _p, _p]),
 }
 
 _lib = None

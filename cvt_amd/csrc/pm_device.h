@@ -170,9 +170,11 @@ __device__ __forceinline__ float4 viewprop(float4 n) {
   return r;
 }
 
-// nd_state_transform, pmhstereo.cl:183-186
+// nd_state_transform, pmhstereo.cl:183-186: (a x + b y) + c with the
+// product a x fused into the first sum (the contraction OpenCL's default
+// FP_CONTRACT permits for this expression)
 __device__ __forceinline__ float transform(float4 s, float x, float y) {
-  return fmaf(s.x, x, fmaf(s.y, y, s.z));
+  return fmaf(s.x, x, s.y * y) + s.z;
 }
 
 __device__ __forceinline__ float4 one_minus(float4 s) {
@@ -294,9 +296,10 @@ __device__ __forceinline__ float smooth_dist(float4 s, float4 sm, float x,
   return ((dx * (dg * dx) + dy * (dg * dy)) + dz * (dg * dz)) + dd * (dw * dd);
 }
 
-// gray value used by the support weight: dot(v, (0.2126, 0.7152, 0.0722))
+// gray value used by the support weight: the built-in
+// dot(v, (0.2126, 0.7152, 0.0722)) as a chain of fused multiply-adds
 __device__ __forceinline__ float gray3(float x, float y, float z) {
-  return (x * 0.2126f + y * 0.7152f) + z * 0.0722f;
+  return fmaf(z, 0.0722f, fmaf(y, 0.7152f, x * 0.2126f));
 }
 
 // exp_f for x <= 0 without branches: the same operation sequence, the

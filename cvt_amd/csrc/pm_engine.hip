@@ -277,11 +277,13 @@ __device__ float cost_eval(const Lane& L, float4 s, float mincost,
     const int qy = L.y + dy;  // uniform
     if (qy < 0 || qy >= L.H) continue;
     if (!live) continue;
-    const float rowc = fmaf(s.y, (float)qy, s.z);
+    const float rowb = s.y * (float)qy;  // transform(), b y per row
     const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * L.TS;
     const float4* trow = L.trow0 + (dy + R) * L.TW + R;
-    for (int dx = dx0; dx <= dx1; ++dx) {
-      const float xq = fmaf(s.x, (float)(L.x + dx), rowc);
+    // tap column as a float, stepped by 1 (exact below 2^24)
+    float qx = L.xf + (float)dx0;
+    for (int dx = dx0; dx <= dx1; ++dx, qx += 1.0f) {
+      const float xq = fmaf(s.x, qx, rowb) + s.z;
       if (xq < 0.0f || xq >= L.Wf) continue;
       const float4 t1 = trow[dx];
       const uint32_t m1 = __float_as_uint(t1.w);
@@ -343,18 +345,18 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
       const int qy = L.y + dy;  // uniform
       if (qy < 0 || qy >= L.H) continue;
       const float py = (float)qy;
-      const float rowc = fmaf(s.y, py, s.z);
+      const float rowb = s.y * py;  // transform(), b y per row
       const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * L.TS;
       const float4* trow = L.trow0 + (dy + R) * L.TW + R;
-      for (int dx = dx0; dx <= dx1; ++dx) {
-        const float px = (float)(L.x + dx);
+      float px = L.xf + (float)dx0;  // stepped by 1, exact below 2^24
+      for (int dx = dx0; dx <= dx1; ++dx, px += 1.0f) {
         const float4 t1 = trow[dx];
         const float v0 = t1.x - L.tc.x, v1 = t1.y - L.tc.y,
                     v2 = t1.z - L.tc.z;
         const uint32_t ga = __float_as_uint(t1.w);
         const float w1 = support_weight(v0, v1, v2, L.cw);
         const float k1 = (w1 * 0.125f) * 0.33f, k2 = w1 * 0.875f;
-        const float xq = fmaf(s.x, px, rowc);
+        const float xq = fmaf(s.x, px, rowb) + s.z;
         Sample sm, s0, sp;
         // The samples at x'-1, x' and x'+1 read the four texels from
         // floor(xc)-1 on, xc = x' clamped to [-1, W]:
@@ -394,12 +396,16 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
         const float C = klt_term(s0, v0, v1, v2, ga, k1, k2);
         const float Cxp = klt_term(sp, v0, v1, v2, ga, k1, k2);
         const float Cxm = klt_term(sm, v0, v1, v2, ga, k1, k2);
-        const float curv = (Cxp + Cxm) - 2.0f * C;
+        // (Cxp + Cxm) - 2 C: the product is exact, so the fma rounds the
+        // same difference
+        const float curv = fmaf(-2.0f, C, Cxp + Cxm);
         if (curv > 0.0f) {
+          // J = J + w1 (Cxm - Cxp) 0.5 p, the last product fused
+          // (pmhstereo.cl:394)
           const float gj = (w1 * (Cxm - Cxp)) * 0.5f;
-          J0 = J0 + gj * px;
-          J1 = J1 + gj * py;
-          J2 = J2 + gj * 1.0f;
+          J0 = fmaf(gj, px, J0);
+          J1 = fmaf(gj, py, J1);
+          J2 = J2 + gj;
           const float h = w1 * curv;
           H00 = H00 + h * (px * px);
           H01 = H01 + h * (px * py);

@@ -14,8 +14,8 @@ scenes the benchmark configurations describe:
   disparity) winning; right pixels no cell maps to (occluded or outside)
   are uniform noise; then its own Gaussian noise.
 
-``evaluate_error`` restates ``Disparity::evaluateError``
-(cvt/vision/Disparity.cpp:201-233).
+The error metrics of the generated pairs are cvt's Disparity ones, in
+``cvt_amd.disparity``.
 """
 import numpy as np
 
@@ -129,20 +129,3 @@ def make_pair(W, H, cells=8, dmax=32.0, channels=1, seed=0, noise=0.01):
         right = np.concatenate([right, one], -1)
     return (left.astype(np.float32), right.astype(np.float32),
             gt.astype(np.float32))
-
-
-def evaluate_error(disp, gt, abs_thres=1.0, rel_thres=0.0):
-    """Disparity::evaluateError (cvt/vision/Disparity.cpp:201-233): over
-    pixels with gt > 0, an error is |gt - d| > abs and |gt - d|/|gt| > rel;
-    returns (error, valid, all) counts (valid = estimate > 0)."""
-    disp = np.asarray(disp, np.float32)
-    gt = np.asarray(gt, np.float32)
-    m = gt > 0
-    diff = np.abs(gt[m] - disp[m])
-    err = (diff > abs_thres) & ((diff / np.abs(gt[m])) > rel_thres)
-    return int(err.sum()), int((disp[m] > 0).sum()), int(m.sum())
-
-
-def bad_pixel_rate(disp, gt, abs_thres=1.0):
-    e, _, a = evaluate_error(disp, gt, abs_thres, 0.0)
-    return e / max(a, 1)

@@ -12,7 +12,10 @@
  * Restatement choices where the reference is implementation-defined
  * (OpenCL built-ins, FP_CONTRACT, evaluation order):
  *  - exp/sin/cos/acospi: own Cody-Waite reduction + Taylor polynomials;
- *  - nd_state_transform: fmaf(a, x, fmaf(b, y, c));
+ *  - nd_state_transform (a x + b y) + c: fmaf(a, x, b * y) + c, the
+ *    contraction FP_CONTRACT (on by default in OpenCL C) permits;
+ *  - dot() of the support weight: fmaf(z, wz, fmaf(y, wy, x * wx));
+ *  - KLT J = J + g * p as one fmaf per component;
  *  - linear sampling: texel t0 + frac*(t1 - t0) as one fmaf, coordinate
  *    u = x' (the sampler's +0.5/-0.5 half-texel shift cancels);
  *  - x / 0.05f as x * 20.0f (OpenCL permits 2.5 ulp division);
@@ -251,10 +254,11 @@ static f4 viewprop(f4 n)
     return r;
 }
 
-/* nd_state_transform, pmhstereo.cl:183-186 */
+/* nd_state_transform, pmhstereo.cl:183-186: (a x + b y) + c with a x
+ * fused into the first sum */
 static inline float transform(f4 s, float x, float y)
 {
-    return fmaf(s.x, x, fmaf(s.y, y, s.z));
+    return fmaf(s.x, x, s.y * y) + s.z;
 }
 
 /* (1,0,0,0) - s */
@@ -442,10 +446,11 @@ void pmo_to_ref_normal_depth(const float* s, float x, float y, int lr,
  * Image precompute
  * ==================================================================== */
 
-/* dot(v, (0.2126, 0.7152, 0.0722)), left to right */
+/* the built-in dot(v, (0.2126, 0.7152, 0.0722)) of the support weight as
+ * a chain of fused multiply-adds */
 static inline float gray3(float x, float y, float z)
 {
-    return (x * 0.2126f + y * 0.7152f) + z * 0.0722f;
+    return fmaf(z, 0.0722f, fmaf(y, 0.7152f, x * 0.2126f));
 }
 
 /* support weight exp(-pow(|dot(d, gray)|, 1) * w), pmhstereo.cl:701 */
@@ -666,7 +671,9 @@ static f4 klt_at(const float* tex1, const float* tex2, int W, int H, int R,
             if (curv > 0.0f) {
                 float p3[3] = {px, py, 1.0f};
                 float g = (w1 * (Cxm - Cxp)) * 0.5f;
-                for (int i = 0; i < 3; i++) J[i] = J[i] + g * p3[i];
+                /* J = J + g * p, the last product fused
+                 * (pmhstereo.cl:394) */
+                for (int i = 0; i < 3; i++) J[i] = fmaf(g, p3[i], J[i]);
                 float h = w1 * curv;
                 for (int i = 0; i < 3; i++)
                     for (int j = 0; j < 3; j++)

@@ -40,7 +40,7 @@ def gray_at(img, x, y):
 
 
 def weight(d, cw=12.0):
-    g = f32(f32(f32(d[0] * GW[0]) + f32(d[1] * GW[1])) + f32(d[2] * GW[2]))
+    g = fma(d[2], GW[2], fma(d[1], GW[1], f32(d[0] * GW[0])))
     return f32(math.exp(float(f32(-abs(g) * f32(cw)))))
 
 
@@ -67,7 +67,7 @@ def prepare(img, r, cw=12.0):
 
 
 def transform(s, x, y):
-    return fma(s[0], f32(x), fma(s[1], f32(y), s[2]))
+    return f32(fma(s[0], f32(x), f32(s[1] * f32(y))) + s[2])
 
 
 def sample(row, u):
@@ -184,7 +184,7 @@ def klt(tex1, tex2, r, x, y, s, cw=12.0):
                 if curv > 0:
                     p3 = [f32(qx), f32(qy), f32(1.0)]
                     g = f32(f32(w1 * f32(Cm - Cp)) * f32(0.5))
-                    J = [f32(J[i] + f32(g * p3[i])) for i in range(3)]
+                    J = [fma(g, p3[i], J[i]) for i in range(3)]
                     h = f32(w1 * curv)
                     for i in range(3):
                         for j in range(3):

@@ -1,5 +1,6 @@
 """The C ABI library without a GPU: it loads, exports every entry point
-include/pm_stereo.h declares, its headers compile as C and C++, parameters
+include/pm_stereo.h and include/pm_io.h declare, its headers compile as C
+and C++, parameters
 are validated before any device is touched, and a host without a HIP
 device gets PM_ERR_HIP (no CPU fallback)."""
 import ctypes
@@ -12,13 +13,16 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pm_stereo.h")
+IO_HEADER = os.path.join(ROOT, "include", "pm_io.h")
 CXX_HEADER = os.path.join(ROOT, "include", "cvt_amd", "PMHuberStereo.h")
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return set(re.findall(r"\b(pm_[a-z_0-9]+)\s*\(", text))
+    names = set()
+    for h in (HEADER, IO_HEADER):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(pm_[a-z_0-9]+)\s*\(", text))
+    return names
 
 
 @pytest.fixture(scope="module")
@@ -29,7 +33,7 @@ def native():
 
 def test_library_exports_every_declared_symbol(native):
     decl = declared_functions()
-    assert len(decl) >= 20
+    assert len(decl) >= 27
     assert decl == set(native.SIGNATURES), decl ^ set(native.SIGNATURES)
     L = native.lib()
     for name in decl:
@@ -118,15 +122,18 @@ def test_missing_library_raises(native, monkeypatch):
 
 C_PROGRAM = r"""
 #include "pm_stereo.h"
+#include "pm_io.h"
 #include <stdio.h>
 int main(void) {
     pm_params p;
     pm_ctx* ctx = 0;
+    int n = 0;
     pm_params_default(&p);
     p.radius = 0;
     int rc = pm_create(&p, &ctx);
     printf("%d %d %s\n", p.disparity_max == 64.0f, rc, pm_last_error(ctx));
     pm_destroy(ctx);
+    if (pm_default_histogram(0, NULL, &n) != PM_OK || n != 15) return 2;
     return rc == PM_ERR_ARG ? 0 : 1;
 }
 """
@@ -156,6 +163,7 @@ def test_cxx_header_compiles(tmp_path):
         pytest.skip("g++ not available")
     src = tmp_path / "t.cpp"
     src.write_text('#include "cvt_amd/PMHuberStereo.h"\n'
+                   '#include "cvt_amd/Disparity.h"\n'
                    "int main() { cvt_amd::Image im(4, 3, 4);\n"
                    "  return im.stride() == 16 ? 0 : 1; }\n")
     subprocess.run([gxx, "-std=c++17", "-Wall", "-Wextra", "-Werror",

@@ -47,6 +47,26 @@ def test_transform_definition(oracle):
     s = np.array([0.75, -0.125, 3.5, 0.0], np.float32)
     assert oracle.transform(s, 10.0, 4.0) == np.float32(0.75 * 10 - 0.125 * 4
                                                          + 3.5)
+    # (a x + b y) + c with a x fused into the first sum: b y rounds, the
+    # fused sum rounds once, then c is added.  Over random planes this
+    # differs from the unfused sum and from b y + c first.
+    rng = np.random.default_rng(3)
+    f64 = np.float64
+    diff_unfused = diff_other = 0
+    for _ in range(400):
+        a, b = np.float32(rng.normal(1, 0.3)), np.float32(rng.normal(0, 0.3))
+        c = np.float32(rng.normal(0, 50))
+        x, y = np.float32(rng.integers(0, 1920)), np.float32(rng.integers(0, 1080))
+        by = np.float32(b * y)
+        want = np.float32(np.float32(f64(a) * f64(x) + f64(by)) + c)
+        s = np.array([a, b, c, 0.0], np.float32)
+        assert oracle.transform(s, x, y) == want
+        unfused = np.float32(np.float32(np.float32(a * x) + by) + c)
+        other = np.float32(f64(a) * f64(x) +
+                           f64(np.float32(f64(b) * f64(y) + f64(c))))
+        diff_unfused += want != unfused
+        diff_other += want != other
+    assert diff_unfused > 0 and diff_other > 0
 
 
 def test_fronto_parallel_zero_normal(oracle):
