@@ -98,12 +98,74 @@ def cpu_baseline(L, R, gt):
             "ms_per_pair": per_pair * 1e3}
 
 
+# algorithmic HBM bytes of one ROF step per pixel: read u, q, f, tensor
+# (16 B each) and the two-float4 dual p (32 B); write u, q (16 B each) and
+# p (32 B)
+ROF_BYTES_PER_PIXEL = 96 + 64
+
+
+def time_rof_step(ctx):
+    """Average duration of one ROF step: two pm_rof runs of 10 and 50 steps
+    on the same field, timed around the call; the difference removes the
+    transfers and buffer set-up both share."""
+    import numpy as np
+    H, Wd = ctx.shape
+    f = np.random.default_rng(0).normal(0, 1, (H, Wd, 4)).astype(np.float32)
+    lam = np.float32([1.5, 1.5, 1.5, 1.0])
+    ctx.rof(0, f, lam, 2)
+    ts = {}
+    for n in (10, 50):
+        t0 = time.perf_counter()
+        ctx.rof(0, f, lam, n)
+        ts[n] = time.perf_counter() - t0
+    return (ts[50] - ts[10]) / 40 * 1e3
+
+
+def cpu_baseline_huber(L, R):
+    """The oracle on a bounded sample of the PM-Huber pipeline: setup
+    (texels, diffusion tensors, init) and one iteration (two propagates,
+    two smoothings of 40 ROF steps); per pair = setup + 8 x iteration."""
+    import numpy as np
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    tl, tr = O.prepare(L, RADIUS), O.prepare(R, RADIUS)
+    dtl, dtr = O.diffusion_tensor(L), O.diffusion_tensor(R)
+    sl = O.init(tl, tr, RADIUS, DMAX, 1)
+    sr = O.init(tr, tl, RADIUS, DMAX, 0)
+    sml = np.zeros(sl.shape, np.float32)
+    smr = np.zeros(sr.shape, np.float32)
+    t1 = time.perf_counter()
+    th = 1e-4
+    lam = np.float32([th * 10 + 1] * 3 + [1.0])
+    vb2 = O.viewbuf_empty(W, H)
+    sl, vb1 = O.propagate(tl, tr, RADIUS, DMAX, 1, 0, sl, vb2, sr, smooth=sml,
+                          theta=th * 50)
+    sr, vb2 = O.propagate(tr, tl, RADIUS, DMAX, 0, 0, sr, vb1, sl, smooth=smr,
+                          theta=th * 50)
+    sml = O.rof(O.normal_depth(sl, 1.0 / DMAX, 1), dtl, lam, 40)
+    smr = O.rof(O.normal_depth(sr, 1.0 / DMAX, 0), dtr, lam, 40)
+    t2 = time.perf_counter()
+    per_pair = (t1 - t0) + ITERS * (t2 - t1)
+    return {"value": 1.0 / per_pair, "unit": "pairs/s",
+            "cores": O.num_threads(), "kind": "port",
+            "sample": "1280x720 pair, PM-Huber: setup + 1 of %d iterations "
+                      "timed (%.1f s); per pair = setup %.2f s + %d x "
+                      "iteration %.2f s" % (ITERS, t2 - t0, t1 - t0, ITERS,
+                                            t2 - t1),
+            "ms_per_pair": per_pair * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pipeline", choices=["plain", "huber"], default="plain",
+                    help="plain PatchMatch (the headline workload) or the "
+                         "PM-Huber pipeline (theta schedule plus "
+                         "diffusion-tensor ROF smoothing after every "
+                         "iteration)")
     ap.add_argument("--io-steps", type=int, default=2,
                     help="pairs run from and to host memory after the timed "
                          "region (PCIe-inclusive rate)")
@@ -155,8 +217,11 @@ def main():
     dR = torch.from_numpy(R).to(dev)
     dD = torch.empty((H, W), dtype=torch.float32, device=dev)
     stream = torch.cuda.Stream(device=dev)
-    ctx = cvt_amd.Context(radius=RADIUS, disparity_max=DMAX,
-                          stream=stream.cuda_stream)
+    huber = args.pipeline == "huber"
+    ctx = cvt_amd.Context(
+        radius=RADIUS, disparity_max=DMAX, stream=stream.cuda_stream,
+        smoothing=cvt_amd.PM_SMOOTH_HUBER if huber else cvt_amd.PM_SMOOTH_NONE,
+        schedule_iterations=ITERS)
 
     def step(events=None):
         ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
@@ -203,6 +268,8 @@ def main():
         ctx.setup()
         cost_taps, klt_taps = ctx.iterate_counted(ITERS)
         torch.cuda.synchronize()
+        if huber:
+            rof_ms = time_rof_step(ctx)
     ctx.close()
 
     launches = 2 * ITERS
@@ -246,9 +313,26 @@ def main():
                      "cost_taps_per_launch": cost_taps / launches,
                      "klt_taps_per_launch": klt_taps / launches},
     }
+    if huber:
+        # the PM-Huber pipeline: same workload, smoothing after every
+        # iteration; its added kernel is the ROF step, an HBM stencil
+        result["metric"] = ("720p stereo pairs/s, PM-Huber pipeline "
+                            "(PatchMatch + diffusion-tensor ROF smoothing)")
+        result["config"]["variant"] = "huber"
+        result["config"]["workload"] = (
+            "PM-Huber PatchMatch, 1280x720 RGBA f32 pair, 24 planar cells, "
+            "disparity 0-128 px, 9x9 patch, 8 iterations with 2 x 40 ROF "
+            "steps each, LR check 1 px")
+        gbs = ROF_BYTES_PER_PIXEL * W * H / (rof_ms * 1e-3) / 1e9
+        result["roofline"] = {
+            "bound": "hbm", "kernel": "k_rof_step", "achieved": gbs,
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+            "traffic": None, "avg_launch_ms": rof_ms,
+            "iteration_ms": float(np.mean(iter_ms))}
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) on a bounded sample")
-        result["cpu_baseline"] = cpu_baseline(L, R, gt)
+        result["cpu_baseline"] = (cpu_baseline_huber(L, R) if huber
+                                  else cpu_baseline(L, R, gt))
     print(json.dumps(result), flush=True)
     return 0
 

@@ -12,6 +12,7 @@ SO_PATH = os.path.join(HERE, "libcvt_amd_pm.so")
 
 PM_OK, PM_ERR_ARG, PM_ERR_STATE, PM_ERR_HIP, PM_ERR_NOMEM, PM_ERR_IO = range(6)
 PM_PP_NONE, PM_PP_LRCHECK, PM_PP_LRCHECK_FILL = range(3)
+PM_SMOOTH_NONE, PM_SMOOTH_HUBER = range(2)
 
 _p = ctypes.c_void_p
 _i = ctypes.c_int
@@ -23,7 +24,9 @@ class PmParams(ctypes.Structure):
     _fields_ = [("radius", ctypes.c_int),
                 ("disparity_max", ctypes.c_float),
                 ("color_weight", ctypes.c_float),
-                ("theta", ctypes.c_float),
+                ("smoothing", ctypes.c_int),
+                ("theta_min", ctypes.c_float),
+                ("schedule_iterations", ctypes.c_int),
                 ("device", ctypes.c_int),
                 ("stream", ctypes.c_void_p)]
 
@@ -54,6 +57,9 @@ SIGNATURES = {
     "pm_iterate_counted": (_i, [_p, _i, _p, _p]),
     "pm_copy_probe": (_i, [_p, _p, _p, ctypes.c_size_t]),
     "pm_math_selftest": (_i, [_p, _p, _i, _p]),
+    "pm_get_smooth": (_i, [_p, _i, _p]),
+    "pm_get_tensor": (_i, [_p, _i, _p]),
+    "pm_rof": (_i, [_p, _i, _p, _p, _i, _p]),
     # include/pm_io.h
     "pm_io_last_error": (ctypes.c_char_p, []),
     "pm_pfm_info": (_i, [ctypes.c_char_p, _p, _p, _p]),

@@ -1,10 +1,11 @@
 // PMHuberStereo.cpp -- C++ drop-in class over the engine's C ABI
-// (mirrors cvt/vision/PMHuberStereo.cpp:75-143 for plain PatchMatch).
+// (mirrors cvt/vision/PMHuberStereo.cpp:75-143).
 #include "cvt_amd/PMHuberStereo.h"
 
 namespace cvt_amd {
 
-PMHuberStereo::PMHuberStereo(int device) : device_(device) {}
+PMHuberStereo::PMHuberStereo(int device, pm_smoothing smoothing)
+    : device_(device), smoothing_(smoothing) {}
 
 PMHuberStereo::~PMHuberStereo() {
   if (ctx_) pm_destroy(ctx_);
@@ -26,6 +27,7 @@ void PMHuberStereo::configure(size_t patchsize, float depthmax) {
   pm_params_default(&p);
   p.radius = (int)patchsize;
   p.disparity_max = depthmax;
+  p.smoothing = smoothing_;
   p.device = device_;
   pm_ctx* c = nullptr;
   int rc = pm_create(&p, &c);
@@ -89,6 +91,12 @@ void PMHuberStereo::setPlanes(const Image& in, int view, size_t iteration) {
   if (in.width() != w_ || in.height() != h_ || in.channels() != 4)
     throw Exception(PM_ERR_ARG, "setPlanes: W x H RGBA_FLOAT expected");
   check(pm_set_planes(ctx_, view, in.ptr(), (int)iteration));
+}
+
+void PMHuberStereo::smoothField(Image& out, int view) {
+  if (!ctx_) throw Exception(PM_ERR_STATE, "PMHuberStereo: call setup first");
+  out.reallocate(w_, h_, 4);
+  check(pm_get_smooth(ctx_, view, out.ptr()));
 }
 
 void PMHuberStereo::disparity(Image& dmap, DisparityPostProcessing pp,

@@ -1,7 +1,8 @@
 // pm_capi.cpp -- C ABI of the PatchMatch engine (include/pm_stereo.h).
 //
-// Host orchestration of cvt::PMHuberStereo's plain PatchMatch path
-// (cvt/vision/PMHuberStereo.cpp:349-404,406-581,583-644) over the HIP
+// Host orchestration of cvt::PMHuberStereo's PatchMatch path, plain or with
+// the PM-Huber smoothing coupling (cvt/vision/PMHuberStereo.cpp:349-404,
+// 406-581,583-644, cvt/vision/PDOptCL.cpp:134-155,1114-1168) over the HIP
 // kernels in pm_engine.hip.  Every device buffer is owned by the context
 // and sized for the current image pair; all work is queued on one stream.
 #include <hip/hip_runtime.h>
@@ -50,6 +51,16 @@ struct pm_ctx {
   DevBuf scratch4b;     // W*H float4
   DevBuf scratch1;      // W*H float
   DevBuf counters;      // 2 x u64 tap counters of the counting pass
+
+  // PM-Huber coupling (PM_SMOOTH_HUBER only)
+  DevBuf dt[2];         // diffusion tensor per view
+  DevBuf smooth[2];     // smoothed (normal, depth) field per view
+  DevBuf nd;            // (normal, depth) field being smoothed
+  DevBuf rof_u;         // ROF primal scratch
+  DevBuf rof_p[2];      // ROF duals p (two float4 per pixel), ping-pong
+  DevBuf rof_q[2];      // ROF duals q, ping-pong
+  float theta = 0.0f;   // theta of the next iteration
+  int schedule_n = 1;   // N of the theta schedule of the running pm_iterate
 };
 
 namespace {
@@ -122,8 +133,12 @@ int check_params(pm_ctx* c, const pm_params* p) {
     return fail(c, PM_ERR_ARG, "disparity_max must be finite and > 0");
   if (!(p->color_weight >= 0.0f) || !std::isfinite(p->color_weight))
     return fail(c, PM_ERR_ARG, "color_weight must be finite and >= 0");
-  if (!(p->theta >= 0.0f) || !std::isfinite(p->theta))
-    return fail(c, PM_ERR_ARG, "theta must be finite and >= 0");
+  if (p->smoothing != PM_SMOOTH_NONE && p->smoothing != PM_SMOOTH_HUBER)
+    return fail(c, PM_ERR_ARG, "unknown smoothing %d", p->smoothing);
+  if (!(p->theta_min >= 0.0f) || !std::isfinite(p->theta_min))
+    return fail(c, PM_ERR_ARG, "theta_min must be finite and >= 0");
+  if (p->schedule_iterations < 0)
+    return fail(c, PM_ERR_ARG, "schedule_iterations must be >= 0");
   if (p->device < 0)
     return fail(c, PM_ERR_ARG, "device ordinal %d is negative", p->device);
   return PM_OK;
@@ -154,6 +169,16 @@ int alloc_pair(pm_ctx* c, int W, int H, int channels) {
   if ((rc = ensure(c, c->scratch4, n * sizeof(float4)))) return rc;
   if ((rc = ensure(c, c->scratch4b, n * sizeof(float4)))) return rc;
   if ((rc = ensure(c, c->scratch1, n * sizeof(float)))) return rc;
+  if (c->prm.smoothing == PM_SMOOTH_HUBER) {
+    for (int v = 0; v < 2; ++v) {
+      if ((rc = ensure(c, c->dt[v], n * sizeof(float4)))) return rc;
+      if ((rc = ensure(c, c->smooth[v], n * sizeof(float4)))) return rc;
+      if ((rc = ensure(c, c->rof_p[v], 2 * n * sizeof(float4)))) return rc;
+      if ((rc = ensure(c, c->rof_q[v], n * sizeof(float4)))) return rc;
+    }
+    if ((rc = ensure(c, c->nd, n * sizeof(float4)))) return rc;
+    if ((rc = ensure(c, c->rof_u, n * sizeof(float4)))) return rc;
+  }
   c->geo.W = W;
   c->geo.H = H;
   c->geo.R = c->prm.radius;
@@ -232,20 +257,77 @@ int require_ready(pm_ctx* c) {
 // disparityMapAll's per-iteration sequence (PMHuberStereo.cpp:531-543):
 // clear viewbuf1, propagate L (reads viewbuf2, writes viewbuf1), clear
 // viewbuf2, propagate R (reads viewbuf1, writes viewbuf2)
+float4* f4buf(DevBuf& b) { return static_cast<float4*>(b.ptr); }
+
+// PM-Huber constants of disparityMapAll (PMHuberStereo.cpp:416-428)
+constexpr float kHuberAlpha = 24.0f;      // diffusion tensor, beta = 1
+constexpr float kThetaScaleNorm = 10.0f;  // THETASCALE_NORM = THETASCALE_D
+constexpr float kThetaBias = 1.0f;        // THETABIAS_NORM = THETABIAS_D
+constexpr float kThetaScale = 50.0f;      // THETASCALE
+constexpr int kRofIters = 40;             // PDOPTITER
+
+// LAMBDA (PMHuberStereo.cpp:532): the normal channels and nz take
+// theta * 10 + 1, the depth channel 1
+float4 huber_lambda(float theta) {
+  const float l = theta * kThetaScaleNorm + kThetaBias;
+  return make_float4(l, l, l, 1.0f);
+}
+
+// denoiseDiffusionTensorROF_PDD (PDOptCL.cpp:1114-1168): out = f, duals 0,
+// `iters` steps ping-ponging between out and the scratch primal (one more
+// when the count is odd, so that the result lands in out).  f must not
+// alias out.
+int rof(pm_ctx* c, int view, const float4* f, float4 lambda, int iters,
+        float4* out) {
+  const size_t n = c->geo.npix;
+  float4* u[2] = {f4buf(c->rof_u), out};
+  float4* p[2] = {f4buf(c->rof_p[0]), f4buf(c->rof_p[1])};
+  float4* q[2] = {f4buf(c->rof_q[0]), f4buf(c->rof_q[1])};
+  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  PM_HIP(c, hipMemcpyAsync(out, f, n * sizeof(float4),
+                           hipMemcpyDeviceToDevice, c->stream));
+  PM_HIP(c, pm::launch_fill4(c->stream, 2 * n, zero, p[1]));
+  PM_HIP(c, pm::launch_fill4(c->stream, n, zero, q[1]));
+  const float4* dt = f4buf(c->dt[view]);
+  int swap = 0;
+  for (int i = 0; i < iters; ++i) {
+    swap = i & 1;
+    PM_HIP(c, pm::launch_rof_step(c->stream, c->geo, u[1 - swap], p[1 - swap],
+                                  q[1 - swap], f, dt, lambda, u[swap],
+                                  p[swap], q[swap]));
+  }
+  if (!swap)
+    PM_HIP(c, pm::launch_rof_step(c->stream, c->geo, u[0], p[0], q[0], f, dt,
+                                  lambda, u[1], p[1], q[1]));
+  return PM_OK;
+}
+
+// (normal, depth) field of a view's current planes, then smoothed into the
+// view's smooth field (PMHuberStereo.cpp:552-567)
+int smooth_view(pm_ctx* c, int view, const float4* planes, float theta) {
+  PM_HIP(c, pm::launch_normal_depth(c->stream, c->geo, planes,
+                                    1.0f / c->prm.disparity_max,
+                                    view == 0 ? 1 : 0, f4buf(c->nd)));
+  return rof(c, view, f4buf(c->nd), huber_lambda(theta), kRofIters,
+             f4buf(c->smooth[view]));
+}
+
 int one_iteration(pm_ctx* c, unsigned long long* counters = nullptr) {
   const int cur = c->cur, nxt = 1 - c->cur;
   unsigned long long* vbL = static_cast<unsigned long long*>(c->vbuf[0].ptr);
   unsigned long long* vbR = static_cast<unsigned long long*>(c->vbuf[1].ptr);
+  const bool huber = c->prm.smoothing == PM_SMOOTH_HUBER;
+  const float theta = c->theta;
   pm::PropagateArgs a;
   a.dmax = c->prm.disparity_max;
-  a.theta = c->prm.theta;
+  a.theta = huber ? theta * kThetaScale : 0.0f;
   a.iter = c->iteration;
 
   PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vbL));
   a.self = view_bufs(c, 0);
   a.other = view_bufs(c, 1);
   a.old = state_ptr(c, 0, cur);
-  a.smooth = nullptr;
+  a.smooth = huber ? f4buf(c->smooth[0]) : nullptr;
   a.viewin = vbR;
   a.src_other = state_ptr(c, 1, cur);
   a.viewout = vbL;
@@ -257,6 +339,7 @@ int one_iteration(pm_ctx* c, unsigned long long* counters = nullptr) {
   a.self = view_bufs(c, 1);
   a.other = view_bufs(c, 0);
   a.old = state_ptr(c, 1, cur);
+  a.smooth = huber ? f4buf(c->smooth[1]) : nullptr;
   a.viewin = vbL;
   a.src_other = state_ptr(c, 0, nxt);
   a.viewout = vbR;
@@ -264,8 +347,41 @@ int one_iteration(pm_ctx* c, unsigned long long* counters = nullptr) {
   a.lr = 0;
   PM_HIP(c, pm::launch_propagate(c->stream, c->geo, a, counters));
 
+  int rc;
+  if (huber) {
+    if ((rc = smooth_view(c, 0, state_ptr(c, 0, nxt), theta))) return rc;
+    if ((rc = smooth_view(c, 1, state_ptr(c, 1, nxt), theta))) return rc;
+    // theta = (iter / N)^4 + theta_min (PMHuberStereo.cpp:575-576)
+    c->theta = std::pow((float)c->iteration / (float)c->schedule_n, 4.0f) +
+               c->prm.theta_min;
+  }
   c->cur = nxt;
   c->iteration += 1;
+  return PM_OK;
+}
+
+// N of the theta schedule for a pm_iterate call of n iterations
+void start_schedule(pm_ctx* c, int n) {
+  c->schedule_n = c->prm.schedule_iterations > 0
+                      ? c->prm.schedule_iterations
+                      : c->iteration + n;
+  if (c->schedule_n < 1) c->schedule_n = 1;
+}
+
+// PM-Huber part of the setup: diffusion tensors of both images, cleared
+// smooth fields, theta = theta_min (PMHuberStereo.cpp:452-458)
+int huber_setup(pm_ctx* c) {
+  c->theta = c->prm.theta_min;
+  if (c->prm.smoothing != PM_SMOOTH_HUBER) return PM_OK;
+  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int v = 0; v < 2; ++v) {
+    PM_HIP(c, pm::launch_diff_tensor(c->stream, c->geo,
+                                     static_cast<const float*>(c->img[v].ptr),
+                                     c->channels, c->geo.W * c->channels,
+                                     kHuberAlpha, f4buf(c->dt[v])));
+    PM_HIP(c, pm::launch_fill4(c->stream, c->geo.npix, zero,
+                               f4buf(c->smooth[v])));
+  }
   return PM_OK;
 }
 
@@ -302,7 +418,9 @@ void pm_params_default(pm_params* p) {
   p->radius = 5;
   p->disparity_max = 64.0f;
   p->color_weight = 12.0f;
-  p->theta = 0.0f;
+  p->smoothing = PM_SMOOTH_NONE;
+  p->theta_min = 1e-4f;
+  p->schedule_iterations = 0;
   p->device = 0;
   p->stream = nullptr;
 }
@@ -353,12 +471,18 @@ void pm_destroy(pm_ctx* c) {
     release(c->state[v][0]);
     release(c->state[v][1]);
     release(c->vbuf[v]);
+    release(c->dt[v]);
+    release(c->smooth[v]);
+    release(c->rof_p[v]);
+    release(c->rof_q[v]);
   }
   release(c->checked);
   release(c->scratch4);
   release(c->scratch4b);
   release(c->scratch1);
   release(c->counters);
+  release(c->nd);
+  release(c->rof_u);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -391,6 +515,7 @@ int pm_setup(pm_ctx* c) {
   PM_HIP(c, pm::launch_viewbuf_clear(
                 c->stream, c->geo,
                 static_cast<unsigned long long*>(c->vbuf[1].ptr)));
+  if ((rc = huber_setup(c))) return rc;
   c->cur = 0;
   c->iteration = 0;
   c->ready = true;
@@ -423,6 +548,12 @@ int pm_setup_prior(pm_ctx* c, const float* disparity, const float* normals) {
   PM_HIP(c, pm::launch_viewbuf_clear(
                 c->stream, c->geo,
                 static_cast<unsigned long long*>(c->vbuf[1].ptr)));
+  if ((rc = huber_setup(c))) return rc;
+  // the seeded left planes start the left smooth field
+  // (PMHuberStereo.cpp:473-477)
+  if (c->prm.smoothing == PM_SMOOTH_HUBER &&
+      (rc = smooth_view(c, 0, state_ptr(c, 0, 0), c->theta)))
+    return rc;
   PM_HIP(c, hipStreamSynchronize(c->stream));  // host inputs may be freed
   c->cur = 0;
   c->iteration = 0;
@@ -436,6 +567,7 @@ int pm_iterate(pm_ctx* c, int n) {
   int rc;
   if ((rc = require_ready(c)) || (rc = use_device(c))) return rc;
   if (n < 0) return fail(c, PM_ERR_ARG, "negative iteration count %d", n);
+  start_schedule(c, n);
   for (int i = 0; i < n; ++i)
     if ((rc = one_iteration(c))) return rc;
   c->last_pp = -1;
@@ -455,6 +587,7 @@ int pm_iterate_counted(pm_ctx* c, int n, uint64_t* cost_taps,
   unsigned long long* d = static_cast<unsigned long long*>(c->counters.ptr);
   const unsigned long long zero[2] = {0ull, 0ull};
   if ((rc = to_device(c, d, zero, sizeof zero))) return rc;
+  start_schedule(c, n);
   for (int i = 0; i < n; ++i)
     if ((rc = one_iteration(c, d))) return rc;
   unsigned long long h[2];
@@ -632,6 +765,50 @@ int pm_copy_probe(pm_ctx* c, const float* d_src, float* d_dst, size_t bytes) {
                                   reinterpret_cast<float4*>(d_dst),
                                   bytes / 16));
   return PM_OK;
+}
+
+static int require_huber(pm_ctx* c) {
+  if (c->prm.smoothing != PM_SMOOTH_HUBER)
+    return fail(c, PM_ERR_STATE, "the context does not use PM_SMOOTH_HUBER");
+  return PM_OK;
+}
+
+int pm_get_smooth(pm_ctx* c, int view, float* out) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = check_view(c, view)) || (rc = require_ready(c)) ||
+      (rc = require_huber(c)) || (rc = use_device(c)))
+    return rc;
+  if (!out) return fail(c, PM_ERR_ARG, "out is NULL");
+  return to_host(c, out, c->smooth[view].ptr, c->geo.npix * sizeof(float4));
+}
+
+int pm_get_tensor(pm_ctx* c, int view, float* out) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = check_view(c, view)) || (rc = require_ready(c)) ||
+      (rc = require_huber(c)) || (rc = use_device(c)))
+    return rc;
+  if (!out) return fail(c, PM_ERR_ARG, "out is NULL");
+  return to_host(c, out, c->dt[view].ptr, c->geo.npix * sizeof(float4));
+}
+
+int pm_rof(pm_ctx* c, int view, const float* f, const float* lambda,
+           int iters, float* out) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = check_view(c, view)) || (rc = require_ready(c)) ||
+      (rc = require_huber(c)) || (rc = use_device(c)))
+    return rc;
+  if (!f || !lambda || !out) return fail(c, PM_ERR_ARG, "NULL argument");
+  if (iters < 1) return fail(c, PM_ERR_ARG, "iters must be >= 1");
+  const size_t bytes = c->geo.npix * sizeof(float4);
+  float4* d_f = static_cast<float4*>(c->scratch4.ptr);
+  float4* d_out = static_cast<float4*>(c->scratch4b.ptr);
+  if ((rc = to_device(c, d_f, f, bytes))) return rc;
+  const float4 lam = make_float4(lambda[0], lambda[1], lambda[2], lambda[3]);
+  if ((rc = rof(c, view, d_f, lam, iters, d_out))) return rc;
+  return to_host(c, out, d_out, bytes);
 }
 
 int pm_math_selftest(pm_ctx* c, const float* in, int n, float* out) {

@@ -327,6 +327,31 @@ __device__ __forceinline__ float support_weight(float dx, float dy, float dz,
   return exp_nonpos(-fabsf(gray3(dx, dy, dz)) * cw);
 }
 
+// ---------------------------------------------------------------------
+// image access
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+__device__ __forceinline__ float4 fetch_rgba(const float* __restrict__ img,
+                                             int C, int stride, int x, int y) {
+  const float* r = img + (size_t)y * stride;
+  if (C == 1) {
+    const float v = r[x];
+    return mk4(v, v, v, v);  // CL_INTENSITY
+  }
+  return *reinterpret_cast<const float4*>(r + 4 * (size_t)x);
+}
+
+__device__ __forceinline__ float gray_at(const float* __restrict__ img, int W,
+                                         int H, int C, int stride, int x,
+                                         int y) {
+  if (x < 0 || x >= W || y < 0 || y >= H) return 0.0f;  // CLK_ADDRESS_CLAMP
+  const float4 v = fetch_rgba(img, C, stride, x, y);
+  return ((v.x * 0.2126f + v.y * 0.7152f) + v.z * 0.0722f) + v.w * 0.0f;
+}
+
 __device__ __forceinline__ uint32_t signmask4(float4 g) {
   return (__float_as_uint(g.x) >> 31) | ((__float_as_uint(g.y) >> 31) << 1) |
          ((__float_as_uint(g.z) >> 31) << 2) |

@@ -79,10 +79,6 @@ __host__ __forceinline__ size_t block_lds_bytes(int R) {
   return (size_t)(kTX + 2 * R) * (kTY + 2 * R) * sizeof(float4);
 }
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) {
-  return v < lo ? lo : (v > hi ? hi : v);
-}
-
 // sign bits of a gradient as one byte per component: byte k is 0xff when
 // component k has its sign bit set (the tile's .w holds the own view's)
 __device__ __forceinline__ uint32_t expand_signs(uint32_t m) {
@@ -444,24 +440,6 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
 // plus the gradient sign bits
 // ---------------------------------------------------------------------
 constexpr int kBX = 64, kBY = 4;  // simple per-pixel kernels
-
-__device__ __forceinline__ float4 fetch_rgba(const float* __restrict__ img,
-                                             int C, int stride, int x, int y) {
-  const float* r = img + (size_t)y * stride;
-  if (C == 1) {
-    const float v = r[x];
-    return mk4(v, v, v, v);  // CL_INTENSITY
-  }
-  return *reinterpret_cast<const float4*>(r + 4 * (size_t)x);
-}
-
-__device__ __forceinline__ float gray_at(const float* __restrict__ img, int W,
-                                         int H, int C, int stride, int x,
-                                         int y) {
-  if (x < 0 || x >= W || y < 0 || y >= H) return 0.0f;  // CLK_ADDRESS_CLAMP
-  const float4 v = fetch_rgba(img, C, stride, x, y);
-  return ((v.x * 0.2126f + v.y * 0.7152f) + v.z * 0.0722f) + v.w * 0.0f;
-}
 
 __global__ void __launch_bounds__(256)
 k_prepare(const float* __restrict__ img, int W, int H, int C, int stride,

@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _native
 from ._native import (PM_PP_LRCHECK, PM_PP_LRCHECK_FILL, PM_PP_NONE,  # noqa
-                      PmParams)
+                      PM_SMOOTH_HUBER, PM_SMOOTH_NONE, PmParams)
 
 DISPARITY_PP_NONE = PM_PP_NONE
 DISPARITY_PP_LRCHECK = PM_PP_LRCHECK
@@ -53,14 +53,17 @@ class Context:
     """One engine context (pm_create ... pm_destroy)."""
 
     def __init__(self, radius=5, disparity_max=64.0, color_weight=12.0,
-                 theta=0.0, device=0, stream=None):
+                 smoothing=PM_SMOOTH_NONE, theta_min=1e-4,
+                 schedule_iterations=0, device=0, stream=None):
         self._lib = _native.lib()
         p = PmParams()
         self._lib.pm_params_default(ctypes.byref(p))
         p.radius = int(radius)
         p.disparity_max = float(disparity_max)
         p.color_weight = float(color_weight)
-        p.theta = float(theta)
+        p.smoothing = int(smoothing)
+        p.theta_min = float(theta_min)
+        p.schedule_iterations = int(schedule_iterations)
         p.device = int(device)
         p.stream = stream
         h = ctypes.c_void_p()
@@ -233,13 +236,43 @@ class Context:
                                                _ptr(out)))
         return out
 
+    # -- PM-Huber coupling -------------------------------------------------
+    def smooth(self, view=0):
+        """The smoothed (normal, disparity / disparity_max) field a view's
+        accept rule pulls towards (PM_SMOOTH_HUBER)."""
+        H, W = self.shape
+        out = np.empty((H, W, 4), np.float32)
+        self._check(self._lib.pm_get_smooth(self._h, int(view), _ptr(out)))
+        return out
+
+    def tensor(self, view=0):
+        """A view's diffusion tensor (PM_SMOOTH_HUBER)."""
+        H, W = self.shape
+        out = np.empty((H, W, 4), np.float32)
+        self._check(self._lib.pm_get_tensor(self._h, int(view), _ptr(out)))
+        return out
+
+    def rof(self, view, f, lam, iters=40):
+        """`iters` diffusion-tensor ROF steps on f with a view's tensor."""
+        H, W = self.shape
+        f = np.ascontiguousarray(f, np.float32)
+        lam = np.ascontiguousarray(lam, np.float32)
+        assert f.shape == (H, W, 4) and lam.shape == (4,)
+        out = np.empty((H, W, 4), np.float32)
+        self._check(self._lib.pm_rof(self._h, int(view), _ptr(f), _ptr(lam),
+                                     int(iters), _ptr(out)))
+        return out
+
 
 class PMHuberStereo:
-    """cvt::PMHuberStereo (cvt/vision/PMHuberStereo.h:42-110), plain
-    PatchMatch form (theta = 0, no primal-dual smoothing)."""
+    """cvt::PMHuberStereo (cvt/vision/PMHuberStereo.h:42-110).  By default
+    its disparityMap pipeline: PatchMatch coupled to the diffusion-tensor
+    ROF smoothing (PM_SMOOTH_HUBER); smoothing=PM_SMOOTH_NONE gives plain
+    PatchMatch."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, smoothing=PM_SMOOTH_HUBER):
         self.device = device
+        self.smoothing = smoothing
         self._ctx = None
         self._cfg = None
 
@@ -249,6 +282,7 @@ class PMHuberStereo:
             if self._ctx is not None:
                 self._ctx.close()
             self._ctx = Context(radius=cfg[0], disparity_max=cfg[1],
+                                smoothing=self.smoothing,
                                 device=self.device)
             self._cfg = cfg
         return self._ctx

@@ -9,10 +9,10 @@
 // setup-then-iterate interface a caller drives directly.  Errors throw
 // cvt_amd::Exception (the reference throws CVTException).
 //
-// Scope: plain PatchMatch, i.e. theta = 0 and no primal-dual smoothing.
-// The reference's disparityMap additionally couples each iteration to a
-// diffusion-tensor ROF smoothing (PMHuberStereo.cpp:551-567); that
-// coupling is not part of this engine.
+// Pipeline: by default the reference's disparityMap one, PatchMatch coupled
+// to a diffusion-tensor ROF smoothing of the (normal, depth) fields after
+// every iteration (PM_SMOOTH_HUBER, PMHuberStereo.cpp:406-581); constructed
+// with PM_SMOOTH_NONE it runs plain PatchMatch (theta = 0, no smoothing).
 #pragma once
 
 #include <cstddef>
@@ -77,7 +77,8 @@ class Image {
 
 class PMHuberStereo {
  public:
-  explicit PMHuberStereo(int device = 0);
+  explicit PMHuberStereo(int device = 0,
+                         pm_smoothing smoothing = PM_SMOOTH_HUBER);
   ~PMHuberStereo();
   PMHuberStereo(const PMHuberStereo&) = delete;
   PMHuberStereo& operator=(const PMHuberStereo&) = delete;
@@ -91,8 +92,10 @@ class PMHuberStereo {
                     const Image* initdisparity = nullptr,
                     const Image* initnormalmap = nullptr);
 
-  // PMHuberStereo.h:50-51 in its plain-PatchMatch form: the disparity map
-  // scaled by dscale (default 1 / depthmax).
+  // PMHuberStereo.h:50-51: the disparityMap result scaled by dscale
+  // (default 1 / depthmax).  The reference's depthMap runs its own variant
+  // of the pipeline (Huber instead of ROF smoothing, occlusion-aware fill,
+  // PMHuberStereo.cpp:147-347); this one keeps disparityMap's.
   void depthMap(Image& dmap, const Image& left, const Image& right,
                 size_t patchsize, float depthmax, size_t iterations,
                 float dscale = -1.0f,
@@ -110,6 +113,9 @@ class PMHuberStereo {
   // plane field of a view (0 = left, 1 = right) as RGBA (a, b, c, cost)
   void planes(Image& out, int view);
   void setPlanes(const Image& in, int view, size_t iteration);
+  // PM_SMOOTH_HUBER: the smoothed (normal, disparity / depthmax) field of a
+  // view
+  void smoothField(Image& out, int view);
   // LR check at (maxdispdiff px, maxanglediff degrees) + conversion
   void disparity(Image& dmap, DisparityPostProcessing pp,
                  Image* normalmap = nullptr, float maxdispdiff = 1.0f,
@@ -122,6 +128,7 @@ class PMHuberStereo {
   void configure(size_t patchsize, float depthmax);
 
   int device_;
+  pm_smoothing smoothing_;
   pm_ctx* ctx_ = nullptr;
   int radius_ = -1;
   float dmax_ = -1.0f;

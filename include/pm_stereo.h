@@ -49,6 +49,18 @@ typedef enum pm_postproc {
     PM_PP_LRCHECK_FILL = 2
 } pm_postproc;
 
+/* Coupling of the plane fields to a smoothed (normal, depth) field */
+typedef enum pm_smoothing {
+    /* plain PatchMatch: theta = 0, no smoothing */
+    PM_SMOOTH_NONE = 0,
+    /* PM-Huber, disparityMapAll's pipeline (PMHuberStereo.cpp:406-581):
+     * theta rises from theta_min, the accept rule weighs theta * 50 times
+     * the distance to the smoothed field (pmhstereo.cl:846-877), and after
+     * every iteration each view's (normal, depth) field is smoothed by 40
+     * diffusion-tensor ROF steps (PDOptCL.cpp:1114-1168) */
+    PM_SMOOTH_HUBER = 1
+} pm_smoothing;
+
 typedef struct pm_params {
     int radius;          /* patch half-width r: the reference's `patchsize`
                             (PMHuberStereo.h:47; loops -r..r,
@@ -56,8 +68,14 @@ typedef struct pm_params {
     float disparity_max; /* `depthmax`, > 0, default 64 */
     float color_weight;  /* support-weight gain, CENSUS_COLOR_WEIGHT = 12
                             (pmhstereo.cl:47) */
-    float theta;         /* smoothness coupling of the accept rule
-                            (pmhstereo.cl:846-877); 0 = plain PatchMatch */
+    int smoothing;       /* pm_smoothing, default PM_SMOOTH_NONE */
+    float theta_min;     /* PM-Huber: first theta, 1e-4 as disparityMap
+                            passes it (PMHuberStereo.cpp:92) */
+    int schedule_iterations; /* PM-Huber: N of theta = (i / N)^4 + theta_min
+                            after iteration i (PMHuberStereo.cpp:575-576);
+                            0 = the iterations run so far plus the count of
+                            the pm_iterate call (one call after setup gives
+                            the reference's schedule) */
     int device;          /* HIP device ordinal, default 0 */
     void* stream;        /* hipStream_t to run on, NULL = own stream */
 } pm_params;
@@ -87,20 +105,28 @@ int pm_set_images_device(pm_ctx* ctx, const float* d_left,
  * (PMHuberStereo.cpp:445-487): bilateral weights and gradients of both
  * images (bilateralWeightToAlpha :678-684, gradient :646-651), smooth
  * fields cleared, init of both views (init :349-360 ->
- * pmhstereo_init), view buffer cleared; iteration counter = 0. */
+ * pmhstereo_init), view buffer cleared; iteration counter = 0.  With
+ * PM_SMOOTH_HUBER also the diffusion tensors of both images
+ * (PDOptCL::diffusionTensor, PDOptCL.cpp:134-155, alpha 24, beta 1) and
+ * theta = theta_min. */
 int pm_setup(pm_ctx* ctx);
 /* Setup seeding the left view from a prior disparity (> 0 where known)
  * and normal map, PMHuberStereo::init with initdisparity/initnormalmap
  * (PMHuberStereo.cpp:361-378 -> pmhstereo_init_disparity_normal,
  * pmhstereo.cl:802-828).  normals may be NULL: they are then derived from
  * the disparity (pmhstereo_disparity_to_normal, pmhstereo.cl:1604-1637).
+ * With PM_SMOOTH_HUBER the left smooth field starts as the smoothed
+ * (normal, depth) field of the seeded planes (PMHuberStereo.cpp:473-477).
  * Host arrays: disparity W x H, normals W x H x 4. */
 int pm_setup_prior(pm_ctx* ctx, const float* disparity, const float* normals);
 
 /* Run n PatchMatch iterations (left view, then right view), the loop of
  * disparityMapAll (PMHuberStereo.cpp:491-580) with propagate() :381-404
- * -> pmhstereo_propagate_view.  Exactly n iterations (the reference's
- * `iterations |= 1` lives in the C++ wrapper's disparityMap). */
+ * -> pmhstereo_propagate_view; with PM_SMOOTH_HUBER each iteration also
+ * smooths both views' (normal, depth) fields (normalDepth :668-675, then
+ * denoiseDiffusionTensorROF_PDD) and advances theta.  Exactly n iterations
+ * (the reference's `iterations |= 1` lives in the C++ wrapper's
+ * disparityMap). */
 int pm_iterate(pm_ctx* ctx, int n);
 int pm_iteration(const pm_ctx* ctx);
 
@@ -140,6 +166,11 @@ int pm_normals(pm_ctx* ctx, int pp, float* normals);
 /* Wait for all work queued on the context. */
 int pm_sync(pm_ctx* ctx);
 
+/* PM_SMOOTH_HUBER: the smoothed (nx, ny, nz, disparity / disparity_max)
+ * field a view's accept rule currently pulls towards (the reference's
+ * leftsmooth / rightsmooth, PMHuberStereo.cpp:439-440), W x H x 4. */
+int pm_get_smooth(pm_ctx* ctx, int view, float* out);
+
 /* ---- diagnostics (no reference counterpart: they expose intermediate
  * results so the engine can be checked piece by piece) ----------------- */
 /* texel records of a view: W x H x 8 floats (r, g, b, bilw, gx, gy, gxy,
@@ -161,6 +192,13 @@ int pm_iterate_counted(pm_ctx* ctx, int n, uint64_t* cost_taps,
  * calibrated against */
 int pm_copy_probe(pm_ctx* ctx, const float* d_src, float* d_dst,
                   size_t bytes);
+/* PM_SMOOTH_HUBER: a view's diffusion tensor (a, b, c, d), W x H x 4 */
+int pm_get_tensor(pm_ctx* ctx, int view, float* out);
+/* PM_SMOOTH_HUBER: `iters` diffusion-tensor ROF steps on the host field f
+ * (W x H x 4) with a view's tensor and the per-channel lambda[4], as
+ * denoiseDiffusionTensorROF_PDD runs them; out W x H x 4 (host) */
+int pm_rof(pm_ctx* ctx, int view, const float* f, const float* lambda,
+           int iters, float* out);
 /* device evaluation of the engine's scalar math for n inputs:
  * out[5*i .. 5*i+4] = exp, sin, cos, acospi, sqrt of in[i] */
 int pm_math_selftest(pm_ctx* ctx, const float* in, int n, float* out);

@@ -54,6 +54,12 @@ _SIGS = {
     "pmo_fill_depthmap": (None, [_p, _i, _i, _f, _p]),
     "pmo_fill_normalmap": (None, [_p, _i, _i, _p]),
     "pmo_disparity_to_normal": (None, [_p, _i, _i, _p]),
+    "pmo_diffusion_tensor": (None, [_p, _i, _i, _i, _i, _f, _p]),
+    "pmo_normal_depth": (None, [_p, _i, _i, _f, _i, _p]),
+    "pmo_rof_step": (None, [_p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p]),
+    "pmo_rof": (_i, [_p, _p, _p, _i, _i, _i, _p]),
+    "pmo_run_huber": (_i, [_p, _p, _i, _i, _i, _i, _i, _f, _i, _f, _f, _f,
+                           _p, _p, _p, _p, _p]),
     "pmo_num_threads": (_i, []),
 }
 
@@ -325,3 +331,67 @@ def disparity_to_normal(disp):
 
 def num_threads():
     return lib().pmo_num_threads()
+
+
+# ---- PM-Huber smoothing coupling -----------------------------------------
+HUBER_THETA_MIN = 1e-4      # disparityMap's theta_min, PMHuberStereo.cpp:92
+HUBER_ALPHA = 24.0          # PMHuberStereo.cpp:416
+
+
+def diffusion_tensor(img, alpha=HUBER_ALPHA):
+    img, W, H, C, stride = _img_args(img)
+    out = np.zeros((H, W, 4), np.float32)
+    lib().pmo_diffusion_tensor(_ptr(img), W, H, C, stride, float(alpha),
+                               _ptr(out))
+    return out
+
+
+def normal_depth(states, dispscale, lr):
+    H, W = states.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().pmo_normal_depth(_ptr(_f32(states)), W, H, float(dispscale),
+                           int(lr), _ptr(out))
+    return out
+
+
+def rof_step(u, p, q, f, dt, lam):
+    H, W = u.shape[:2]
+    uo = np.zeros((H, W, 4), np.float32)
+    po = np.zeros((H, W, 8), np.float32)
+    qo = np.zeros((H, W, 4), np.float32)
+    lam = np.asarray(lam, np.float32)
+    lib().pmo_rof_step(_ptr(_f32(u)), _ptr(_f32(p)), _ptr(_f32(q)),
+                       _ptr(_f32(f)), _ptr(_f32(dt)), _ptr(lam), W, H,
+                       _ptr(uo), _ptr(po), _ptr(qo))
+    return uo, po, qo
+
+
+def rof(f, dt, lam, iters=40):
+    H, W = f.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lam = np.asarray(lam, np.float32)
+    if lib().pmo_rof(_ptr(_f32(f)), _ptr(_f32(dt)), _ptr(lam), int(iters), W,
+                     H, _ptr(out)):
+        raise ValueError("pmo_rof failed")
+    return out
+
+
+def run_huber(left, right, radius, dmax, iters, theta_min=HUBER_THETA_MIN,
+              maxdisp=1.0, maxdeg=5.0):
+    """PM-Huber pipeline; returns dict of stateL, stateR, disp, smoothL,
+    smoothR."""
+    left, W, H, C, stride = _img_args(left)
+    right = np.ascontiguousarray(right, dtype=np.float32)
+    assert right.shape == left.shape
+    out = {k: np.zeros((H, W, 4), np.float32)
+           for k in ("stateL", "stateR", "smoothL", "smoothR")}
+    out["disp"] = np.zeros((H, W), np.float32)
+    rc = lib().pmo_run_huber(_ptr(left), _ptr(right), W, H, C, stride,
+                             int(radius), float(dmax), int(iters),
+                             float(theta_min), float(maxdisp), float(maxdeg),
+                             _ptr(out["stateL"]), _ptr(out["stateR"]),
+                             _ptr(out["disp"]), _ptr(out["smoothL"]),
+                             _ptr(out["smoothR"]))
+    if rc != 0:
+        raise MemoryError("pmo_run_huber failed")
+    return out

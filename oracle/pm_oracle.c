@@ -1076,6 +1076,174 @@ void pmo_disparity_to_normal(const float* disp, int W, int H, float* normals)
         }
 }
 
+/* ======================================================================
+ * PM-Huber smoothing coupling
+ * ==================================================================== */
+
+/* PDDiffTensorColor / PDDiffTensorGray (PDDiffTensor.cl:25-116), beta = 1
+ * so pow(len, beta) = len.  Gray value: the 4-component dot with
+ * (0.2126, 0.7152, 0.0722, 0) for RGBA images, the intensity itself for
+ * gray ones; CLK_ADDRESS_CLAMP reads 0 outside the image. */
+static float dt_gray(const float* img, int W, int H, int channels,
+                     int stride, int x, int y)
+{
+    if (x < 0 || x >= W || y < 0 || y >= H) return 0.0f;
+    if (channels == 1) return img[(size_t)y * stride + x];
+    return gray_at(img, W, H, channels, stride, x, y);
+}
+
+void pmo_diffusion_tensor(const float* img, int W, int H, int channels,
+                          int stride, float alpha, float* dt)
+{
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            float g = dt_gray(img, W, H, channels, stride, x, y);
+            float dx = dt_gray(img, W, H, channels, stride, x + 1, y) - g;
+            float dy = dt_gray(img, W, H, channels, stride, x, y + 1) - g;
+            float len = sqrtf(dx * dx + dy * dy);
+            float w = pmo_expf(-alpha * len);
+            float* o = dt + ((size_t)y * W + x) * 4;
+            if (len <= 0.001f) {
+                o[0] = 1.0f;
+                o[1] = 0.0f;
+                o[2] = 0.0f;
+                o[3] = 1.0f;
+            } else {
+                dx = dx / len;
+                dy = dy / len;
+                float dxo = dy, dyo = -dx;
+                o[0] = (w * dx) * dx + dxo * dxo;
+                o[1] = (w * dx) * dy + dxo * dyo;
+                o[2] = (w * dy) * dx + dyo * dxo;
+                o[3] = (w * dy) * dy + dyo * dyo;
+            }
+        }
+}
+
+/* pmhstereo_normal_depth, pmhstereo.cl:1384-1402 */
+void pmo_normal_depth(const float* states, int W, int H, float dispscale,
+                      int lr, float* out)
+{
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            size_t p = (size_t)y * W + x;
+            f4 v = to_ref_normal_depth(load4(states, p), (float)x, (float)y,
+                                       lr);
+            v.w = v.w * dispscale;
+            memcpy(out + p * 4, &v, sizeof v);
+        }
+}
+
+#define ROF_SIGMA 4.0f  /* PDDROFDiffTensor.cl:36-39 */
+#define ROF_TAU 0.05f
+#define ROF_ETA 8.0f
+#define ROF_THETA 0.5f  /* PDDROFDiffTensor.cl:41 */
+
+/* the tensor-applied projected dual at pixel (i, j) of one step
+ * (PDDROFDiffTensor.cl:69-98): u read clamped to the edge, tensor and old
+ * dual read as 0 outside the image */
+static void rof_dual(const float* u, const float* p, const float* dt, int W,
+                     int H, int i, int j, float lo[4], float hi[4])
+{
+    int inside = i >= 0 && i < W && j >= 0 && j < H;
+    int ci = i < 0 ? 0 : (i > W - 1 ? W - 1 : i);
+    int cj = j < 0 ? 0 : (j > H - 1 ? H - 1 : j);
+    int ci1 = i + 1 < 0 ? 0 : (i + 1 > W - 1 ? W - 1 : i + 1);
+    int cj1 = j + 1 < 0 ? 0 : (j + 1 > H - 1 ? H - 1 : j + 1);
+    const float* u0 = u + ((size_t)cj * W + ci) * 4;
+    const float* ux = u + ((size_t)cj * W + ci1) * 4;
+    const float* uy = u + ((size_t)cj1 * W + ci) * 4;
+    float D[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float plo[4] = {0.0f, 0.0f, 0.0f, 0.0f}, phi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (inside) {
+        size_t q = (size_t)j * W + i;
+        memcpy(D, dt + q * 4, sizeof D);
+        memcpy(plo, p + q * 8, sizeof plo);
+        memcpy(phi, p + q * 8 + 4, sizeof phi);
+    }
+    float n[4];
+    for (int k = 0; k < 4; k++) {
+        float dx = u0[k] - ux[k], dy = u0[k] - uy[k];
+        float DTdx = D[0] * dx + D[2] * dy;
+        float DTdy = D[1] * dx + D[3] * dy;
+        plo[k] = plo[k] + ROF_SIGMA * DTdx;
+        phi[k] = phi[k] + ROF_SIGMA * DTdy;
+        n[k] = plo[k] * plo[k] + phi[k] * phi[k];
+    }
+    float nxyz = sqrtf((n[0] + n[1]) + n[2]);
+    float proj[4] = {fmaxf(1.0f, nxyz), fmaxf(1.0f, nxyz), fmaxf(1.0f, nxyz),
+                     fmaxf(1.0f, sqrtf(n[3]))};
+    for (int k = 0; k < 4; k++) {
+        plo[k] = plo[k] / proj[k];
+        phi[k] = phi[k] / proj[k];
+        lo[k] = D[0] * plo[k] + D[2] * phi[k];
+        hi[k] = D[1] * plo[k] + D[3] * phi[k];
+    }
+}
+
+/* PDDROFDiffTensor, PDDROFDiffTensor.cl:43-125 */
+void pmo_rof_step(const float* u, const float* p, const float* q,
+                  const float* f, const float* dt, const float lambda[4],
+                  int W, int H, float* u_out, float* p_out, float* q_out)
+{
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            size_t o = (size_t)y * W + x;
+            float Plo[4], Phi[4], Llo[4], Lhi[4], Ulo[4], Uhi[4];
+            rof_dual(u, p, dt, W, H, x, y, Plo, Phi);
+            rof_dual(u, p, dt, W, H, x - 1, y, Llo, Lhi);
+            rof_dual(u, p, dt, W, H, x, y - 1, Ulo, Uhi);
+            for (int k = 0; k < 4; k++) {
+                float div = ((Plo[k] - Llo[k]) + Phi[k]) - Uhi[k];
+                float img = f[o * 4 + k], qq = q[o * 4 + k];
+                float uo = u[o * 4 + k];
+                float un = (uo + ROF_TAU * ((lambda[k] * img - div) - qq)) /
+                           (1.0f + ROF_TAU * lambda[k]);
+                un = un + ROF_THETA * (un - uo);
+                float qn = (qq + ROF_ETA * (un - img)) /
+                           (1.0f + 4.0f * ROF_ETA / lambda[k]);
+                qn = qn + ROF_THETA * (qn - qq);
+                u_out[o * 4 + k] = un;
+                q_out[o * 4 + k] = qn;
+                p_out[o * 8 + k] = Plo[k];
+                p_out[o * 8 + 4 + k] = Phi[k];
+            }
+        }
+}
+
+/* denoiseDiffusionTensorROF_PDD, PDOptCL.cpp:1114-1168: output = input,
+ * duals 0, `iters` steps (one more when the count is odd, so that the
+ * result lands in the output image: never for the 40 the callers use) */
+int pmo_rof(const float* f, const float* dt, const float lambda[4],
+            int iters, int W, int H, float* u_out)
+{
+    size_t n = (size_t)W * H;
+    if (iters < 1) return -1;
+    float* u[2] = {(float*)malloc(n * 16), (float*)malloc(n * 16)};
+    float* p[2] = {(float*)calloc(n * 8, 4), (float*)calloc(n * 8, 4)};
+    float* q[2] = {(float*)calloc(n * 4, 4), (float*)calloc(n * 4, 4)};
+    if (!u[0] || !u[1] || !p[0] || !p[1] || !q[0] || !q[1]) return -1;
+    memcpy(u[1], f, n * 16);
+    int swap = 0;
+    for (int i = 0; i < iters; i++) {
+        swap = i & 1;
+        pmo_rof_step(u[1 - swap], p[1 - swap], q[1 - swap], f, dt, lambda,
+                     W, H, u[swap], p[swap], q[swap]);
+    }
+    if (!swap)
+        pmo_rof_step(u[0], p[0], q[0], f, dt, lambda, W, H, u[1], p[1],
+                     q[1]);
+    memcpy(u_out, u[1], n * 16);
+    for (int k = 0; k < 2; k++) {
+        free(u[k]);
+        free(p[k]);
+        free(q[k]);
+    }
+    return 0;
+}
+
 int pmo_num_threads(void)
 {
 #ifdef _OPENMP
@@ -1138,6 +1306,84 @@ int pmo_run(const float* left, const float* right, int W, int H,
     free(sL[1]);
     free(sR[0]);
     free(sR[1]);
+    free(vb1);
+    free(vb2);
+    free(chk);
+    return 0;
+}
+
+/* PM-Huber: disparityMapAll (PMHuberStereo.cpp:406-581) with its
+ * constants (:416-428): alpha 24, beta 1, lambda = (10 theta + 1,
+ * 10 theta + 1, 10 theta + 1, 1), theta * 50 in the accept rule, 40 ROF
+ * steps; theta starts at theta_min and becomes (i / N)^4 + theta_min after
+ * iteration i (:575-576, Math::pow = powf). */
+int pmo_run_huber(const float* left, const float* right, int W, int H,
+                  int channels, int stride, int radius, float dmax,
+                  int iters, float theta_min, float maxdispdiff,
+                  float maxdeg, float* stateL, float* stateR, float* disp,
+                  float* smoothL, float* smoothR)
+{
+    const float cw = 12.0f;
+    size_t n = (size_t)W * H;
+    float* texL = (float*)malloc(n * 8 * sizeof(float));
+    float* texR = (float*)malloc(n * 8 * sizeof(float));
+    float* sL[2] = {(float*)malloc(n * 16), (float*)malloc(n * 16)};
+    float* sR[2] = {(float*)malloc(n * 16), (float*)malloc(n * 16)};
+    float* dtL = (float*)malloc(n * 16);
+    float* dtR = (float*)malloc(n * 16);
+    float* smL = (float*)calloc(n * 4, 4);
+    float* smR = (float*)calloc(n * 4, 4);
+    float* nd = (float*)malloc(n * 16);
+    uint64_t* vb1 = (uint64_t*)malloc(n * 8);
+    uint64_t* vb2 = (uint64_t*)malloc(n * 8);
+    float* chk = (float*)malloc(n * 16);
+    if (!texL || !texR || !sL[0] || !sL[1] || !sR[0] || !sR[1] || !dtL ||
+        !dtR || !smL || !smR || !nd || !vb1 || !vb2 || !chk)
+        return -1;
+    pmo_prepare(left, W, H, channels, stride, radius, cw, texL);
+    pmo_prepare(right, W, H, channels, stride, radius, cw, texR);
+    pmo_diffusion_tensor(left, W, H, channels, stride, 24.0f, dtL);
+    pmo_diffusion_tensor(right, W, H, channels, stride, 24.0f, dtR);
+    pmo_init(texL, texR, W, H, radius, cw, dmax, 1, sL[0]);
+    pmo_init(texR, texL, W, H, radius, cw, dmax, 0, sR[0]);
+    pmo_viewbuf_clear(vb2, W, H);
+    float theta = theta_min;
+    int cur = 0;
+    for (int it = 0; it < iters; it++) {
+        const float lam = theta * 10.0f + 1.0f;
+        const float lambda[4] = {lam, lam, lam, 1.0f};
+        pmo_viewbuf_clear(vb1, W, H);
+        pmo_propagate(texL, texR, W, H, radius, cw, dmax, theta * 50.0f, 1,
+                      it, sL[cur], smL, vb2, sR[cur], vb1, sL[1 - cur]);
+        pmo_viewbuf_clear(vb2, W, H);
+        pmo_propagate(texR, texL, W, H, radius, cw, dmax, theta * 50.0f, 0,
+                      it, sR[cur], smR, vb1, sL[1 - cur], vb2, sR[1 - cur]);
+        pmo_normal_depth(sL[1 - cur], W, H, 1.0f / dmax, 1, nd);
+        if (pmo_rof(nd, dtL, lambda, 40, W, H, smL)) return -1;
+        pmo_normal_depth(sR[1 - cur], W, H, 1.0f / dmax, 0, nd);
+        if (pmo_rof(nd, dtR, lambda, 40, W, H, smR)) return -1;
+        theta = powf((float)it / (float)iters, 4.0f) + theta_min;
+        cur = 1 - cur;
+    }
+    if (stateL) memcpy(stateL, sL[cur], n * 16);
+    if (stateR) memcpy(stateR, sR[cur], n * 16);
+    if (smoothL) memcpy(smoothL, smL, n * 16);
+    if (smoothR) memcpy(smoothR, smR, n * 16);
+    if (disp) {
+        pmo_consistency(sL[cur], sR[cur], W, H, maxdispdiff, maxdeg, chk);
+        pmo_to_disparity(chk, W, H, 1.0f, 1, disp);
+    }
+    free(texL);
+    free(texR);
+    free(sL[0]);
+    free(sL[1]);
+    free(sR[0]);
+    free(sR[1]);
+    free(dtL);
+    free(dtR);
+    free(smL);
+    free(smR);
+    free(nd);
     free(vb1);
     free(vb2);
     free(chk);

@@ -127,6 +127,40 @@ int pmo_run(const float* left, const float* right, int W, int H,
             float maxdispdiff, float maxdeg, float* stateL, float* stateR,
             float* disp, float* cost_trace);
 
+/* ---- PM-Huber smoothing coupling (theta > 0) ---------------------------
+ * PDDiffTensorColor / PDDiffTensorGray (cvt/cl/kernel/PDDiffTensor.cl:
+ * 25-116) with beta = 1: dt = W x H x 4 tensor (a, b, c, d) from the
+ * forward differences of the gray image (zero outside). */
+void pmo_diffusion_tensor(const float* img, int W, int H, int channels,
+                          int stride, float alpha, float* dt);
+/* pmhstereo_normal_depth (pmhstereo.cl:1384-1402): (nx, ny, nz,
+ * disparity * dispscale) of every plane, W x H x 4 */
+void pmo_normal_depth(const float* states, int W, int H, float dispscale,
+                      int lr, float* out);
+/* one PDDROFDiffTensor step (cvt/cl/kernel/PDDROFDiffTensor.cl:43-125):
+ * u, q, f, dt are W x H x 4, p is W x H x 8 (lo, hi); writes u_out,
+ * p_out, q_out (p_out holds the tensor-applied projected dual, as the
+ * kernel stores it) */
+void pmo_rof_step(const float* u, const float* p, const float* q,
+                  const float* f, const float* dt, const float lambda[4],
+                  int W, int H, float* u_out, float* p_out, float* q_out);
+/* denoiseDiffusionTensorROF_PDD (cvt/vision/PDOptCL.cpp:1114-1168): u = f,
+ * p = q = 0, then `iters` steps; u_out W x H x 4 */
+int pmo_rof(const float* f, const float* dt, const float lambda[4],
+            int iters, int W, int H, float* u_out);
+
+/* ---- whole PM-Huber pipeline: disparityMapAll (PMHuberStereo.cpp:
+ * 406-581) with one pyramid level: exactly `iters` iterations (the caller
+ * makes the count odd as disparityMap does), theta from theta_min,
+ * diffusion-tensor ROF smoothing of both views after every iteration;
+ * then the LR check and disparity conversion.  smoothL / smoothR (may be
+ * NULL) receive the final smoothed (normal, depth) fields. */
+int pmo_run_huber(const float* left, const float* right, int W, int H,
+                  int channels, int stride, int radius, float dmax,
+                  int iters, float theta_min, float maxdispdiff,
+                  float maxdeg, float* stateL, float* stateR, float* disp,
+                  float* smoothL, float* smoothR);
+
 int pmo_num_threads(void);
 
 #ifdef __cplusplus

@@ -2,7 +2,8 @@
 // drives cvt::PMHuberStereo::disparityMap (cvt/vision/PMHuberStereo.h:47-48).
 //
 // usage: disparity_map W H C radius dmax iterations pp left.f32 right.f32
-//                      disp_out.f32 planes_out.f32 normals_out.f32
+//                      disp_out.f32 planes_out.f32 normals_out.f32 [plain]
+// ("plain": PatchMatch without the PM-Huber smoothing coupling)
 // Images are raw row-major float32 files; outputs are W x H (disparity)
 // and W x H x 4 (left plane field after the iterations, normal map).
 #include <cstdio>
@@ -30,7 +31,7 @@ static bool write_raw(const char* path, const cvt_amd::Image& im) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 13) {
+  if (argc != 13 && argc != 14) {
     std::fprintf(stderr, "usage: %s W H C radius dmax iterations pp left "
                          "right disp planes normals\n", argv[0]);
     return 2;
@@ -48,7 +49,8 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "cannot read the input images\n");
       return 2;
     }
-    cvt_amd::PMHuberStereo pm;
+    const bool plain = argc == 14 && std::string(argv[13]) == "plain";
+    cvt_amd::PMHuberStereo pm(0, plain ? PM_SMOOTH_NONE : PM_SMOOTH_HUBER);
     pm.disparityMap(dmap, left, right, radius, dmax, iters,
                     static_cast<cvt_amd::DisparityPostProcessing>(pp),
                     &normals);

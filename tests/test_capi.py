@@ -9,6 +9,7 @@ import re
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -56,14 +57,17 @@ def test_params_default_are_reference_constants(native):
     assert p.radius == 5                 # cvt::PMHuberStereo patchsize
     assert p.disparity_max == 64.0
     assert p.color_weight == 12.0        # CENSUS_COLOR_WEIGHT
-    assert p.theta == 0.0                # plain PatchMatch
+    assert p.smoothing == native.PM_SMOOTH_NONE     # plain PatchMatch
+    assert p.theta_min == np.float32(1e-4)          # PMHuberStereo.cpp:92
+    assert p.schedule_iterations == 0
     assert p.device == 0 and not p.stream
 
 
 @pytest.mark.parametrize("field,value", [
     ("radius", 0), ("radius", 17), ("disparity_max", 0.0),
     ("disparity_max", float("nan")), ("disparity_max", float("inf")),
-    ("color_weight", -1.0), ("theta", -0.5), ("device", -1)])
+    ("color_weight", -1.0), ("theta_min", -0.5), ("theta_min", float("nan")),
+    ("smoothing", 2), ("schedule_iterations", -1), ("device", -1)])
 def test_create_rejects_bad_parameters(native, field, value):
     L = native.lib()
     p = native.PmParams()

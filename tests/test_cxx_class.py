@@ -37,8 +37,8 @@ def bits(a):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("pp", [1, 2])
-def test_disparity_map_matches_oracle(tmp_path, oracle, synth, pp):
+@pytest.mark.parametrize("pp,mode", [(1, "huber"), (2, "huber"), (1, "plain")])
+def test_disparity_map_matches_oracle(tmp_path, oracle, synth, pp, mode):
     O = oracle
     W, H, C, R, dmax, iters = 72, 40, 4, 3, 14.0, 2
     L, Rt, _ = synth.make_pair(W, H, cells=4, dmax=dmax, channels=C, seed=21)
@@ -46,10 +46,12 @@ def test_disparity_map_matches_oracle(tmp_path, oracle, synth, pp):
     paths = [str(tmp_path / n) for n in ("l", "r", "d", "p", "n")]
     L.astype(np.float32).tofile(paths[0])
     Rt.astype(np.float32).tofile(paths[1])
-    print("cxx driver: %dx%d C%d r=%d pp=%d" % (W, H, C, R, pp), flush=True)
+    print("cxx driver: %dx%d C%d r=%d pp=%d %s" % (W, H, C, R, pp, mode),
+          flush=True)
     r = subprocess.run([exe, str(W), str(H), str(C), str(R), str(dmax),
-                        str(iters), str(pp)] + paths, capture_output=True,
-                       text=True, timeout=120)
+                        str(iters), str(pp)] + paths +
+                       (["plain"] if mode == "plain" else []),
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     # disparityMap makes the iteration count odd (PMHuberStereo.cpp:490)
     assert "iterations run: 3" in r.stdout
@@ -57,7 +59,12 @@ def test_disparity_map_matches_oracle(tmp_path, oracle, synth, pp):
     d = np.fromfile(paths[2], np.float32).reshape(H, W)
     planes = np.fromfile(paths[3], np.float32).reshape(H, W, 4)
     normals = np.fromfile(paths[4], np.float32).reshape(H, W, 4)
-    ref = O.run(L, Rt, R, dmax, iters | 1, 1.0, 5.0)
+    # the class's default is the reference's disparityMap pipeline
+    # (PM-Huber, theta_min 1e-4)
+    if mode == "huber":
+        ref = O.run_huber(L, Rt, R, dmax, iters | 1, 1e-4, 1.0, 5.0)
+    else:
+        ref = O.run(L, Rt, R, dmax, iters | 1, 1.0, 5.0)
     np.testing.assert_array_equal(bits(planes), bits(ref["stateL"]))
     chk = O.consistency(ref["stateL"], ref["stateR"], 1.0, 5.0)
     want = O.to_disparity(chk) if pp == 1 else O.fill_depthmap(chk)

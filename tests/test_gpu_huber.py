@@ -1,0 +1,135 @@
+"""Parity of the PM-Huber smoothing coupling (PM_SMOOTH_HUBER) with the
+oracle, through the C ABI: diffusion tensor, ROF smoothing, the coupled
+iterations with the theta schedule, and the prior-seeded smooth field.
+Floats are compared bit for bit."""
+import numpy as np
+import pytest
+
+import cvt_amd
+
+pytestmark = pytest.mark.gpu
+
+F = np.float32
+
+
+def bits(a):
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).copy()
+    u[np.isnan(a)] = 0x7FC00000
+    return u
+
+
+def assert_same(got, want, what):
+    d = np.argwhere(bits(got) != bits(want))
+    assert d.size == 0, "%s: %d differ, first %s: %r vs %r" % (
+        what, len(d), tuple(d[0]), got[tuple(d[0])], want[tuple(d[0])])
+
+
+def huber_ctx(r, dmax, **kw):
+    return cvt_amd.Context(radius=r, disparity_max=dmax,
+                           smoothing=cvt_amd.PM_SMOOTH_HUBER, **kw)
+
+
+@pytest.fixture(scope="module")
+def pair(synth):
+    return synth.make_pair(131, 77, cells=5, dmax=20.0, channels=4, seed=12)
+
+
+@pytest.fixture(scope="module")
+def pair_gray(synth):
+    return synth.make_pair(96, 64, cells=4, dmax=16.0, channels=1, seed=13)
+
+
+@pytest.mark.parametrize("which", ["rgba", "gray"])
+def test_tensor_bitexact(oracle, pair, pair_gray, which):
+    L, R, _ = pair if which == "rgba" else pair_gray
+    print("tensor", which, L.shape, flush=True)
+    with huber_ctx(3, 16.0) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for view, img in ((0, L), (1, R)):
+            assert_same(ctx.tensor(view), oracle.diffusion_tensor(img),
+                        "tensor view %d" % view)
+
+
+@pytest.mark.parametrize("iters", [40, 3])
+def test_rof_bitexact(oracle, pair, iters):
+    L, R, _ = pair
+    H, W = L.shape[:2]
+    rng = np.random.default_rng(iters)
+    f = rng.normal(0, 1, (H, W, 4)).astype(np.float32)
+    lam = F([1.3, 1.3, 1.3, 1.0])
+    print("rof", iters, flush=True)
+    with huber_ctx(3, 20.0) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        got = ctx.rof(1, f, lam, iters)
+    want = oracle.rof(f, oracle.diffusion_tensor(R), lam, iters)
+    assert_same(got, want, "rof")
+
+
+def test_huber_iterations_bitexact(oracle, pair):
+    L, R, _ = pair
+    r, dmax, n = 3, 20.0, 3
+    print("huber pipeline", flush=True)
+    with huber_ctx(r, dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(n)
+        sl, sr = ctx.planes(0), ctx.planes(1)
+        ml, mr = ctx.smooth(0), ctx.smooth(1)
+        d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)
+    ref = oracle.run_huber(L, R, r, dmax, n, 1e-4, 1.0, 5.0)
+    assert_same(sl, ref["stateL"], "left planes")
+    assert_same(sr, ref["stateR"], "right planes")
+    assert_same(ml, ref["smoothL"], "left smooth field")
+    assert_same(mr, ref["smoothR"], "right smooth field")
+    assert_same(d, ref["disp"], "disparity")
+
+
+def test_schedule_over_split_calls(pair):
+    """With schedule_iterations = N, iterating 2 + 1 gives what one call of
+    3 gives (theta = (i / N)^4 + theta_min after iteration i)."""
+    L, R, _ = pair
+    out = []
+    for split in ((3,), (2, 1)):
+        with huber_ctx(3, 20.0, schedule_iterations=3) as ctx:
+            ctx.set_images(L, R)
+            ctx.setup()
+            for k in split:
+                ctx.iterate(k)
+            out.append((ctx.planes(0), ctx.smooth(1)))
+    assert_same(out[1][0], out[0][0], "planes")
+    assert_same(out[1][1], out[0][1], "smooth")
+
+
+def test_prior_seeds_the_smooth_field(oracle, pair):
+    L, R, gt = pair
+    r, dmax = 3, 20.0
+    H, W = gt.shape
+    prior = gt.copy()
+    prior[::3, ::5] = 0.0                 # some pixels without a prior
+    print("huber prior", flush=True)
+    with huber_ctx(r, dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup_prior(prior)
+        sm = ctx.smooth(0)
+        sl = ctx.planes(0)
+    tl, tr = oracle.prepare(L, r), oracle.prepare(R, r)
+    want_sl = oracle.init_prior(tl, tr, r, dmax, 1, prior,
+                                oracle.disparity_to_normal(prior))
+    assert_same(sl, want_sl, "seeded planes")
+    lam_v = F(F(1e-4) * F(10) + F(1))
+    want = oracle.rof(oracle.normal_depth(want_sl, F(1) / F(dmax), 1),
+                      oracle.diffusion_tensor(L),
+                      F([lam_v, lam_v, lam_v, 1.0]), 40)
+    assert_same(sm, want, "seeded smooth field")
+
+
+def test_plain_context_has_no_smooth_field(pair):
+    L, R, _ = pair
+    with cvt_amd.Context(radius=2, disparity_max=8.0) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        with pytest.raises(cvt_amd.PMError):
+            ctx.smooth(0)
