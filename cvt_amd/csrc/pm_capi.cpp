@@ -630,6 +630,15 @@ int pm_set_planes(pm_ctx* c, int view, const float* in, int iteration) {
                                  vbR));
   PM_HIP(c, hipStreamSynchronize(c->stream));
   c->iteration = iteration;
+  // PM-Huber: theta of the next iteration when the schedule length is
+  // fixed (PMHuberStereo.cpp:575-576); the smooth fields stay as they are
+  if (c->prm.smoothing == PM_SMOOTH_HUBER && c->prm.schedule_iterations > 0)
+    c->theta = iteration == 0
+                   ? c->prm.theta_min
+                   : std::pow((float)(iteration - 1) /
+                                  (float)c->prm.schedule_iterations,
+                              4.0f) +
+                         c->prm.theta_min;
   c->last_pp = -1;
   return PM_OK;
 }

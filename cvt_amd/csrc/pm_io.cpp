@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -116,7 +117,7 @@ extern "C" {
 
 const char* pm_io_last_error(void) { return g_err.c_str(); }
 
-int pm_pfm_info(const char* path, int* W, int* H, int* channels) {
+int pm_pfm_info(const char* path, int* W, int* H, int* channels) try {
   if (!W || !H || !channels) return io_fail(PM_ERR_ARG, "output is NULL");
   PfmFile f;
   int rc = parse_pfm(path, &f);
@@ -125,11 +126,13 @@ int pm_pfm_info(const char* path, int* W, int* H, int* channels) {
   *H = f.H;
   *channels = f.file_channels == 1 ? 1 : 4;
   return PM_OK;
+} catch (const std::bad_alloc&) {
+  return io_fail(PM_ERR_NOMEM, "out of host memory");
 }
 
 // PFMLoader::load, parseBinary, parseBinarySwap (PFMLoader.cpp:107-205)
 int pm_pfm_read(const char* path, float* out, int W, int H, int channels,
-                int stride) {
+                int stride) try {
   if (!out) return io_fail(PM_ERR_ARG, "out is NULL");
   PfmFile f;
   int rc = parse_pfm(path, &f);
@@ -138,13 +141,13 @@ int pm_pfm_read(const char* path, float* out, int W, int H, int channels,
   if (W != f.W || H != f.H || channels != C)
     return io_fail(PM_ERR_ARG, "%s holds %dx%d with %d channels, not %dx%d "
                    "with %d", path, f.W, f.H, C, W, H, channels);
-  if (stride < W * C)
-    return io_fail(PM_ERR_ARG, "stride %d < width*channels %d", stride, W * C);
+  if ((long long)stride < (long long)W * C)
+    return io_fail(PM_ERR_ARG, "stride %d < width*channels", stride);
   // little-endian samples when the scale is negative on a little-endian
   // host, byte-swapped samples otherwise (PFMLoader.cpp:122-125)
   const bool swap = !(host_little_endian() && f.scale < 0.0f);
   const unsigned char* src = f.bytes.data() + f.data;
-  const int n = W * f.file_channels;
+  const size_t n = (size_t)W * f.file_channels;
   for (int row = 0; row < H; ++row) {  // file rows run bottom to top
     float* dst = out + (size_t)(H - 1 - row) * stride;
     for (int x = 0; x < W; ++x) {
@@ -159,18 +162,19 @@ int pm_pfm_read(const char* path, float* out, int W, int H, int channels,
     }
   }
   return PM_OK;
+} catch (const std::bad_alloc&) {
+  return io_fail(PM_ERR_NOMEM, "out of host memory");
 }
 
 // PFMSaver::save (plugins/isaver/PFM/PFMSaver.cpp:117-181)
 int pm_pfm_write(const char* path, const float* data, int W, int H,
-                 int channels, int stride) {
+                 int channels, int stride) try {
   if (!path || !data) return io_fail(PM_ERR_ARG, "path or data is NULL");
   if (W < 1 || H < 1) return io_fail(PM_ERR_ARG, "invalid size %dx%d", W, H);
   if (channels != 1 && channels != 4)
     return io_fail(PM_ERR_ARG, "channels must be 1 (GRAY) or 4 (RGBA)");
-  if (stride < W * channels)
-    return io_fail(PM_ERR_ARG, "stride %d < width*channels %d", stride,
-                   W * channels);
+  if ((long long)stride < (long long)W * channels)
+    return io_fail(PM_ERR_ARG, "stride %d < width*channels", stride);
   std::ofstream out(path, std::ios::binary | std::ios::trunc);
   if (!out) return io_fail(PM_ERR_IO, "cannot open %s for writing", path);
   out << (channels == 4 ? "PF" : "Pf") << "\n" << W << " " << H << "\n"
@@ -188,6 +192,8 @@ int pm_pfm_write(const char* path, const float* data, int W, int H,
   out.close();
   if (!out) return io_fail(PM_ERR_IO, "write to %s failed", path);
   return PM_OK;
+} catch (const std::bad_alloc&) {
+  return io_fail(PM_ERR_NOMEM, "out of host memory");
 }
 
 // Disparity::evaluateError (cvt/vision/Disparity.cpp:201-233)

@@ -135,7 +135,9 @@ int pm_get_planes(pm_ctx* ctx, int view, float* out);
 /* Replace a view's plane field; `iteration` sets the counter that seeds
  * the next iteration's draws.  The view-propagation buffer the next left
  * step reads is rebuilt from the right field (empty when iteration is 0),
- * so a run resumes exactly from the two fields and the counter. */
+ * so a run resumes exactly from the two fields and the counter.  With
+ * PM_SMOOTH_HUBER the smooth fields are kept and theta follows the counter
+ * when schedule_iterations is set. */
 int pm_set_planes(pm_ctx* ctx, int view, const float* in, int iteration);
 
 /* Cost of the given planes at every pixel of a view, mincost = 1e10
