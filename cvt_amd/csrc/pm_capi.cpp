@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 #include <string>
 
 #include "pm_engine.h"
@@ -718,14 +719,30 @@ int pm_get_texels(pm_ctx* c, int view, float* out) {
       (rc = use_device(c)))
     return rc;
   if (!out) return fail(c, PM_ERR_ARG, "out is NULL");
-  // the W texels of each row, without the edge padding
-  const size_t rec = 2 * sizeof(float4);
-  const char* src =
-      static_cast<const char*>(c->tex[view].ptr) + pm::kTexPadL * rec;
-  PM_HIP(c, hipMemcpy2DAsync(out, c->geo.W * rec, src, c->geo.TS * rec,
-                             c->geo.W * rec, c->geo.H, hipMemcpyDeviceToHost,
-                             c->stream));
+  // the W texels of each row without the edge padding: colour plane, then
+  // gradient plane, interleaved on the host into (colour, gradient) pairs
+  const int W = c->geo.W, H = c->geo.H;
+  const size_t rec = sizeof(float4), npix = (size_t)W * H;
+  const size_t plane = (size_t)c->geo.TS * H * rec;
+  std::vector<float4> tmp;
+  try {
+    tmp.resize(2 * npix);
+  } catch (const std::bad_alloc&) {
+    return fail(c, PM_ERR_NOMEM, "out of host memory");
+  }
+  for (int k = 0; k < 2; ++k) {
+    const char* src = static_cast<const char*>(c->tex[view].ptr) + k * plane +
+                      pm::kTexPadL * rec;
+    PM_HIP(c, hipMemcpy2DAsync(tmp.data() + k * npix, W * rec, src,
+                               c->geo.TS * rec, W * rec, H,
+                               hipMemcpyDeviceToHost, c->stream));
+  }
   PM_HIP(c, hipStreamSynchronize(c->stream));
+  float4* o = reinterpret_cast<float4*>(out);
+  for (size_t p = 0; p < npix; ++p) {
+    o[2 * p] = tmp[p];
+    o[2 * p + 1] = tmp[npix + p];
+  }
   return PM_OK;
 }
 

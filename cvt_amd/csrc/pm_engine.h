@@ -9,12 +9,14 @@
 
 namespace pm {
 
-// Device buffers of one view.  A texel record is two float4:
-// (r, g, b, bilw) then (gx, gy, gxy, gyx).  Texel rows are padded with
-// copies of their edge texels, kTexPadL on the left and kTexPadR on the
-// right, so that every linear sample the kernels form (coordinates
-// clamped to [-1, W]) reads existing records without clamping its
-// indices: texel (x, y) is record y * tex_stride(W) + kTexPadL + x.
+// Device buffers of one view.  The texels live in two planes of
+// tex_stride(W) * H float4 records each: the colour plane (r, g, b, bilw)
+// and, right after it, the gradient plane (gx, gy, gxy, gyx).  Texel rows
+// are padded with copies of their edge texels, kTexPadL on the left and
+// kTexPadR on the right, so that every linear sample the kernels form
+// (coordinates clamped to [-1, W]) reads existing records without
+// clamping its indices: texel (x, y) is record y * tex_stride(W) +
+// kTexPadL + x of each plane.
 // sgn[y * W + x] = sign bits of the gradient (bit k = signbit of
 // component k).
 struct ViewBufs {

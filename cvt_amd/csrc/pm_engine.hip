@@ -8,9 +8,10 @@
 //  - the block stages its own-view tile (pixels plus an r halo: colour and
 //    gradient sign bits) in LDS once; support weights are recomputed per
 //    tap from that tile instead of being streamed from HBM;
-//  - the other view is sampled straight from its 32-byte texel records
-//    (two adjacent records per linear sample), which keeps a wavefront's
-//    loads within few cache lines of the vector L1.
+//  - the other view is sampled straight from its colour and gradient
+//    planes (two adjacent 16-byte records of each per linear sample), so
+//    a wavefront's gather of one plane spans half the cache lines that
+//    interleaved 32-byte texels would.
 // The cost loops run wave-uniformly over patch rows (one image row per
 // wavefront) and every lane stays resident (no early return): lanes are
 // masked with flags, and kernels reach the block barrier with all threads.
@@ -114,7 +115,8 @@ struct Lane {
                         // trow0[(dy + R) * TW + dx + R]
   int TW;
   int TS;               // texel row stride (records, with the edge pads)
-  const float4* __restrict__ tex2;  // other view's texel (0, 0), global
+  size_t GP;            // offset of the gradient plane (records)
+  const float4* __restrict__ tex2;  // other view's colour record (0, 0)
 };
 
 // Block prologue shared by the cost-evaluating kernels: stages the own-view
@@ -132,7 +134,7 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
     const int gx = x0 - R + i % TW, gy = y0 - R + i / TW;
     float4 v = mk4(0.0f, 0.0f, 0.0f, 0.0f);
     if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-      const float4 a = self.tex[2 * ((size_t)gy * g.TS + kTexPadL + gx)];
+      const float4 a = self.tex[(size_t)gy * g.TS + kTexPadL + gx];
       v = mk4(a.x, a.y, a.z,
               __uint_as_float(expand_signs(self.sgn[(size_t)gy * W + gx])));
     }
@@ -155,19 +157,20 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
   L.p = L.valid ? (size_t)L.y * W + L.x : 0;
   L.TW = TW;
   L.TS = g.TS;
+  L.GP = (size_t)g.TS * g.H;
   L.trow0 = smem + ly * TW + threadIdx.x;
-  L.tex2 = other.tex + 2 * kTexPadL;
+  L.tex2 = other.tex + kTexPadL;
   L.tc = L.trow0[R * TW + R];
   L.bilw = L.valid
-               ? self.tex[2 * ((size_t)L.y * g.TS + kTexPadL + L.x)].w
+               ? self.tex[(size_t)L.y * g.TS + kTexPadL + L.x].w
                : 1.0f;
   return L;
 }
 
 // Linear sample (CLK_FILTER_LINEAR, CLK_ADDRESS_CLAMP_TO_EDGE,
 // pmhstereo.cl:71) of the other view at column u of a texel row:
-// t0 + a*(t1 - t0).  Adjacent 32-byte texel records keep a wavefront's
-// loads within few cache lines.
+// t0 + a*(t1 - t0), from adjacent records of the colour and the gradient
+// plane.
 struct Sample {
   float r, g, b;
   float4 grad;
@@ -177,11 +180,13 @@ struct Texel {
   float4 A, G;  // (r, g, b, bilw), gradient
 };
 
+// texel i of a row: colour record from the colour plane, gradient record
+// from the gradient plane gp records further
 __device__ __forceinline__ Texel texel(const float4* __restrict__ srow,
-                                       uint32_t i) {
+                                       uint32_t i, size_t gp) {
   Texel t;
-  t.A = srow[2 * i];
-  t.G = srow[2 * i + 1];
+  t.A = srow[i];
+  t.G = srow[gp + i];
   return t;
 }
 
@@ -219,16 +224,16 @@ __device__ __forceinline__ Sample sample(const Lane& L,
   const Coord c = coord_of(L, u);
   const uint32_t i0 = (uint32_t)clampi(c.f, 0, L.W - 1);
   const uint32_t i1 = (uint32_t)clampi(c.f + 1, 0, L.W - 1);
-  return lerp_texels(texel(srow, i0), texel(srow, i1), c.a);
+  return lerp_texels(texel(srow, i0, L.GP), texel(srow, i1, L.GP), c.a);
 }
 
 // sample() for 0 <= u < W: no coordinate clamp, and the second texel of
 // u in [W-1, W) is the right edge pad, a copy of texel W-1
 __device__ __forceinline__ Sample sample_inside(const float4* __restrict__ srow,
-                                                float u) {
+                                                size_t gp, float u) {
   const float fl = floorf(u);
   const uint32_t i0 = (uint32_t)(int)fl;
-  return lerp_texels(texel(srow, i0), texel(srow, i0 + 1u), u - fl);
+  return lerp_texels(texel(srow, i0, gp), texel(srow, i0 + 1u, gp), u - fl);
 }
 
 // 8 x the gradient sign mismatches between expanded sign bytes m1x and
@@ -274,7 +279,7 @@ __device__ float cost_eval(const Lane& L, float4 s, float mincost,
     if (qy < 0 || qy >= L.H) continue;
     if (!live) continue;
     const float rowb = s.y * (float)qy;  // transform(), b y per row
-    const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * L.TS;
+    const float4* __restrict__ srow = L.tex2 + (size_t)qy * L.TS;
     const float4* trow = L.trow0 + (dy + R) * L.TW + R;
     // tap column as a float, stepped by 1 (exact below 2^24)
     float qx = L.xf + (float)dx0;
@@ -285,7 +290,7 @@ __device__ float cost_eval(const Lane& L, float4 s, float mincost,
       const uint32_t m1 = __float_as_uint(t1.w);
       const float w = support_weight(t1.x - L.tc.x, t1.y - L.tc.y,
                                      t1.z - L.tc.z, L.cw);
-      const Sample s2 = sample_inside(srow, xq);
+      const Sample s2 = sample_inside(srow, L.GP, xq);
       const float c0 = fminf(fabsf(s2.r - t1.x), 0.05f) * 20.0f;
       const float c1 = fminf(fabsf(s2.g - t1.y), 0.05f) * 20.0f;
       const float c2 = fminf(fabsf(s2.b - t1.z), 0.05f) * 20.0f;
@@ -342,7 +347,7 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
       if (qy < 0 || qy >= L.H) continue;
       const float py = (float)qy;
       const float rowb = s.y * py;  // transform(), b y per row
-      const float4* __restrict__ srow = L.tex2 + 2 * (size_t)qy * L.TS;
+      const float4* __restrict__ srow = L.tex2 + (size_t)qy * L.TS;
       const float4* trow = L.trow0 + (dy + R) * L.TW + R;
       float px = L.xf + (float)dx0;  // stepped by 1, exact below 2^24
       for (int dx = dx0; dx <= dx1; ++dx, px += 1.0f) {
@@ -371,16 +376,17 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
         if (__all(flp == fl + 1.0f)) {
           const float a = xc - fl;
           const float ap = xp1 - flp;
-          const float4* __restrict__ t = srow + 2 * ((int)fl - 1);
+          const float4* __restrict__ t = srow + ((int)fl - 1);
+          const float4* __restrict__ tg = t + L.GP;
           Texel t0, t1, t2, t3;
           t0.A = t[0];
-          t0.G = t[1];
-          t1.A = t[2];
-          t1.G = t[3];
-          t2.A = t[4];
-          t2.G = t[5];
-          t3.A = t[6];
-          t3.G = t[7];
+          t1.A = t[1];
+          t2.A = t[2];
+          t3.A = t[3];
+          t0.G = tg[0];
+          t1.G = tg[1];
+          t2.G = tg[2];
+          t3.G = tg[3];
           sm = lerp_texels(t0, t1, a);
           s0 = lerp_texels(t1, t2, a);
           sp = lerp_texels(t2, t3, ap);
@@ -468,19 +474,20 @@ k_prepare(const float* __restrict__ img, int W, int H, int C, int stride,
   const float gym = gray_at(img, W, H, C, stride, x, y - 1);
   const float4 grad = mk4(gxp - g, gyp - g, gxp - gxm, gyp - gym);
   const float4 col = mk4(c.x, c.y, c.z, wsum);
-  float4* row = tex + 2 * ((size_t)y * tex_stride(W) + kTexPadL);
-  row[2 * x] = col;
-  row[2 * x + 1] = grad;
+  const size_t gp = (size_t)tex_stride(W) * H;  // gradient plane offset
+  float4* row = tex + (size_t)y * tex_stride(W) + kTexPadL;
+  row[x] = col;
+  row[gp + x] = grad;
   // edge pads: copies of the first and last texel of the row
   if (x == 0)
     for (int i = 1; i <= kTexPadL; ++i) {
-      row[-2 * i] = col;
-      row[-2 * i + 1] = grad;
+      row[-i] = col;
+      row[gp - i] = grad;
     }
   if (x == W - 1)
     for (int i = 1; i <= kTexPadR; ++i) {
-      row[2 * (x + i)] = col;
-      row[2 * (x + i) + 1] = grad;
+      row[x + i] = col;
+      row[gp + x + i] = grad;
     }
   sgn[p] = (uint8_t)signmask4(grad);
 }
