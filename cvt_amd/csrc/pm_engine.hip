@@ -227,13 +227,13 @@ __device__ __forceinline__ Sample sample(const Lane& L,
   return lerp_texels(texel(srow, i0, L.GP), texel(srow, i1, L.GP), c.a);
 }
 
-// sample() for 0 <= u < W: no coordinate clamp, and the second texel of
-// u in [W-1, W) is the right edge pad, a copy of texel W-1
-__device__ __forceinline__ Sample sample_inside(const float4* __restrict__ srow,
-                                                size_t gp, float u) {
-  const float fl = floorf(u);
-  const uint32_t i0 = (uint32_t)(int)fl;
-  return lerp_texels(texel(srow, i0, gp), texel(srow, i0 + 1u, gp), u - fl);
+// texel at record p of the colour plane and p + gp of the gradient plane
+__device__ __forceinline__ Texel texel_at(const float4* __restrict__ p,
+                                          size_t gp) {
+  Texel t;
+  t.A = p[0];
+  t.G = p[gp];
+  return t;
 }
 
 // 8 x the gradient sign mismatches between expanded sign bytes m1x and
@@ -290,7 +290,12 @@ __device__ float cost_eval(const Lane& L, float4 s, float mincost,
       const uint32_t m1 = __float_as_uint(t1.w);
       const float w = support_weight(t1.x - L.tc.x, t1.y - L.tc.y,
                                      t1.z - L.tc.z, L.cw);
-      const Sample s2 = sample_inside(srow, L.GP, xq);
+      // 0 <= xq < W: no coordinate clamp; the second texel of xq in
+      // [W-1, W) is the right edge pad, a copy of texel W-1
+      const float fq = floorf(xq);
+      const float4* __restrict__ tq = srow + (int)fq;
+      const Sample s2 =
+          lerp_texels(texel_at(tq, L.GP), texel_at(tq + 1, L.GP), xq - fq);
       const float c0 = fminf(fabsf(s2.r - t1.x), 0.05f) * 20.0f;
       const float c1 = fminf(fabsf(s2.g - t1.y), 0.05f) * 20.0f;
       const float c2 = fminf(fabsf(s2.b - t1.z), 0.05f) * 20.0f;
@@ -349,6 +354,10 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
       const float rowb = s.y * py;  // transform(), b y per row
       const float4* __restrict__ srow = L.tex2 + (size_t)qy * L.TS;
       const float4* trow = L.trow0 + (dy + R) * L.TW + R;
+      // texel window srow[kf - 1 .. kf + 2] of the previous fast-path tap;
+      // a slope near 1 slides it by one texel per tap
+      int kf = INT_MIN;
+      Texel tw0, tw1, tw2, tw3;
       float px = L.xf + (float)dx0;  // stepped by 1, exact below 2^24
       for (int dx = dx0; dx <= dx1; ++dx, px += 1.0f) {
         const float4 t1 = trow[dx];
@@ -376,20 +385,23 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
         if (__all(flp == fl + 1.0f)) {
           const float a = xc - fl;
           const float ap = xp1 - flp;
-          const float4* __restrict__ t = srow + ((int)fl - 1);
-          const float4* __restrict__ tg = t + L.GP;
-          Texel t0, t1, t2, t3;
-          t0.A = t[0];
-          t1.A = t[1];
-          t2.A = t[2];
-          t3.A = t[3];
-          t0.G = tg[0];
-          t1.G = tg[1];
-          t2.G = tg[2];
-          t3.G = tg[3];
-          sm = lerp_texels(t0, t1, a);
-          s0 = lerp_texels(t1, t2, a);
-          sp = lerp_texels(t2, t3, ap);
+          const int f = (int)fl;  // -1 <= f <= W
+          if (f != kf) {
+            if (f == kf + 1) {
+              tw0 = tw1;
+              tw1 = tw2;
+              tw2 = tw3;
+            } else {
+              tw0 = texel_at(srow + (f - 1), L.GP);
+              tw1 = texel_at(srow + f, L.GP);
+              tw2 = texel_at(srow + (f + 1), L.GP);
+            }
+            tw3 = texel_at(srow + (f + 2), L.GP);
+            kf = f;
+          }
+          sm = lerp_texels(tw0, tw1, a);
+          s0 = lerp_texels(tw1, tw2, a);
+          sp = lerp_texels(tw2, tw3, ap);
         } else {
           sm = sample(L, srow, xq - 1.0f);
           s0 = sample(L, srow, xq);
