@@ -60,6 +60,7 @@ SIGNATURES = {
     "pm_get_smooth": (_i, [_p, _i, _p]),
     "pm_get_tensor": (_i, [_p, _i, _p]),
     "pm_rof": (_i, [_p, _i, _p, _p, _i, _p]),
+    "pm_rof_device": (_i, [_p, _i, _p, _p, _i, _p]),
     # include/pm_io.h
     "pm_io_last_error": (ctypes.c_char_p, []),
     "pm_pfm_info": (_i, [ctypes.c_char_p, _p, _p, _p]),

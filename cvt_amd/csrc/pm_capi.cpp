@@ -7,6 +7,7 @@
 // and sized for the current image pair; all work is queued on one stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -285,21 +286,26 @@ int rof(pm_ctx* c, int view, const float4* f, float4 lambda, int iters,
   float4* p[2] = {f4buf(c->rof_p[0]), f4buf(c->rof_p[1])};
   float4* q[2] = {f4buf(c->rof_q[0]), f4buf(c->rof_q[1])};
   const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  PM_HIP(c, hipMemcpyAsync(out, f, n * sizeof(float4),
-                           hipMemcpyDeviceToDevice, c->stream));
-  PM_HIP(c, pm::launch_fill4(c->stream, 2 * n, zero, p[1]));
-  PM_HIP(c, pm::launch_fill4(c->stream, n, zero, q[1]));
   const float4* dt = f4buf(c->dt[view]);
-  int swap = 0;
-  for (int i = 0; i < iters; ++i) {
-    swap = i & 1;
-    PM_HIP(c, pm::launch_rof_step(c->stream, c->geo, u[1 - swap], p[1 - swap],
-                                  q[1 - swap], f, dt, lambda, u[swap],
-                                  p[swap], q[swap]));
+  // The reference ping-pongs one step per launch and adds a step when the
+  // count is odd, so that the result lands in the output image.  Here a
+  // launch runs up to kRofStepsMax steps; the launches alternate between
+  // the two buffer sets, starting from the set that makes the last one
+  // write the output.
+  const int steps = iters + (iters & 1);
+  const int launches = (steps + pm::kRofStepsMax - 1) / pm::kRofStepsMax;
+  int src = (launches & 1) ? 0 : 1;
+  PM_HIP(c, hipMemcpyAsync(u[src], f, n * sizeof(float4),
+                           hipMemcpyDeviceToDevice, c->stream));
+  PM_HIP(c, pm::launch_fill4(c->stream, 2 * n, zero, p[src]));
+  PM_HIP(c, pm::launch_fill4(c->stream, n, zero, q[src]));
+  for (int done = 0; done < steps; done += pm::kRofStepsMax) {
+    const int k = std::min(pm::kRofStepsMax, steps - done);
+    PM_HIP(c, pm::launch_rof_steps(c->stream, c->geo, k, u[src], p[src],
+                                   q[src], f, dt, lambda, u[1 - src],
+                                   p[1 - src], q[1 - src]));
+    src = 1 - src;
   }
-  if (!swap)
-    PM_HIP(c, pm::launch_rof_step(c->stream, c->geo, u[0], p[0], q[0], f, dt,
-                                  lambda, u[1], p[1], q[1]));
   return PM_OK;
 }
 
@@ -835,6 +841,25 @@ int pm_rof(pm_ctx* c, int view, const float* f, const float* lambda,
   const float4 lam = make_float4(lambda[0], lambda[1], lambda[2], lambda[3]);
   if ((rc = rof(c, view, d_f, lam, iters, d_out))) return rc;
   return to_host(c, out, d_out, bytes);
+}
+
+int pm_rof_device(pm_ctx* c, int view, const float* d_f, const float* lambda,
+                  int iters, float* d_out) {
+  if (!c) return PM_ERR_ARG;
+  int rc;
+  if ((rc = check_view(c, view)) || (rc = require_ready(c)) ||
+      (rc = require_huber(c)) || (rc = use_device(c)))
+    return rc;
+  if (!d_f || !lambda || !d_out) return fail(c, PM_ERR_ARG, "NULL argument");
+  if (iters < 1) return fail(c, PM_ERR_ARG, "iters must be >= 1");
+  const size_t bytes = c->geo.npix * sizeof(float4);
+  const char* f0 = reinterpret_cast<const char*>(d_f);
+  const char* o0 = reinterpret_cast<const char*>(d_out);
+  if (f0 < o0 + bytes && o0 < f0 + bytes)
+    return fail(c, PM_ERR_ARG, "d_f and d_out overlap");
+  const float4 lam = make_float4(lambda[0], lambda[1], lambda[2], lambda[3]);
+  return rof(c, view, reinterpret_cast<const float4*>(d_f), lam, iters,
+             reinterpret_cast<float4*>(d_out));
 }
 
 int pm_math_selftest(pm_ctx* c, const float* in, int n, float* out) {

@@ -101,8 +101,8 @@ hipError_t launch_fill_normalmap(hipStream_t st, const Geometry& g,
                                  const float4* states, float4* normals);
 
 // PM-Huber coupling: diffusion tensor of an image (alpha, beta = 1),
-// (normal, depth) field of a plane field, one diffusion-tensor ROF step
-// (p holds two float4 per pixel), and a float4 fill
+// (normal, depth) field of a plane field, diffusion-tensor ROF steps, and
+// a float4 fill
 hipError_t launch_diff_tensor(hipStream_t st, const Geometry& g,
                               const float* img, int channels, int stride,
                               float alpha, float4* dt);
@@ -111,10 +111,13 @@ hipError_t launch_normal_depth(hipStream_t st, const Geometry& g,
                                const float4* states, float dispscale, int lr,
                                float4* out);
 
-hipError_t launch_rof_step(hipStream_t st, const Geometry& g, const float4* u,
-                           const float4* p, const float4* q, const float4* f,
-                           const float4* dt, float4 lambda, float4* u_out,
-                           float4* p_out, float4* q_out);
+// nsteps <= kRofStepsMax diffusion-tensor ROF steps in one launch (u, q,
+// f, the tensor: one float4 per pixel; p: two)
+constexpr int kRofStepsMax = 2;
+hipError_t launch_rof_steps(hipStream_t st, const Geometry& g, int nsteps,
+                            const float4* u, const float4* p, const float4* q,
+                            const float4* f, const float4* dt, float4 lambda,
+                            float4* u_out, float4* p_out, float4* q_out);
 
 hipError_t launch_fill4(hipStream_t st, size_t n, float4 v, float4* out);
 

@@ -72,26 +72,13 @@ struct Dual {
   float lo[4], hi[4];
 };
 
-// tensor-applied projected dual of one step at (i, j) (PDDROFDiffTensor.cl:
-// 69-98): u clamped to the edge, tensor and old dual 0 outside the image
-__device__ __forceinline__ Dual rof_dual(const float4* __restrict__ u,
-                                         const float4* __restrict__ p,
-                                         const float4* __restrict__ dt, int W,
-                                         int H, int i, int j) {
-  const bool inside = i >= 0 && i < W && j >= 0 && j < H;
-  const int ci = clampi(i, 0, W - 1), cj = clampi(j, 0, H - 1);
-  const int ci1 = clampi(i + 1, 0, W - 1), cj1 = clampi(j + 1, 0, H - 1);
-  const float4 u0 = u[(size_t)cj * W + ci];
-  const float4 ux = u[(size_t)cj * W + ci1];
-  const float4 uy = u[(size_t)cj1 * W + ci];
-  float4 D = mk4(0.0f, 0.0f, 0.0f, 0.0f);
-  float4 plo4 = D, phi4 = D;
-  if (inside) {
-    const size_t q = (size_t)j * W + i;
-    D = dt[q];
-    plo4 = p[2 * q];
-    phi4 = p[2 * q + 1];
-  }
+// tensor-applied projected dual of one step (PDDROFDiffTensor.cl:69-98)
+// from u at the pixel, its right and its lower neighbour (clamped to the
+// edge), the tensor D and the old dual (both 0 outside the image)
+__device__ __forceinline__ Dual dual_of(const float4& u0, const float4& ux,
+                                        const float4& uy, const float4& D,
+                                        const float4& plo4,
+                                        const float4& phi4) {
   Dual r;
   float n[4];
 #pragma unroll
@@ -115,42 +102,140 @@ __device__ __forceinline__ Dual rof_dual(const float4* __restrict__ u,
   return r;
 }
 
-// one PDDROFDiffTensor step (PDDROFDiffTensor.cl:43-125).  Each pixel forms
-// its own dual and those of its left and upper neighbours (the kernel's
-// LDS tile shares them; the values are the same).  The dual written back
-// is the tensor-applied one, as the kernel stores it.
-__global__ void __launch_bounds__(256)
-k_rof_step(int W, int H, const float4* __restrict__ u,
-           const float4* __restrict__ p, const float4* __restrict__ q,
-           const float4* __restrict__ f, const float4* __restrict__ dt,
-           float4 lambda, float4* __restrict__ u_out,
-           float4* __restrict__ p_out, float4* __restrict__ q_out) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= W || y >= H) return;
-  const size_t o = (size_t)y * W + x;
-  const Dual P = rof_dual(u, p, dt, W, H, x, y);
-  const Dual Lf = rof_dual(u, p, dt, W, H, x - 1, y);
-  const Dual Up = rof_dual(u, p, dt, W, H, x, y - 1);
-  const float4 fi = f[o], qi = q[o], ui = u[o];
+// primal and q update of one pixel (PDDROFDiffTensor.cl:100-124) from its
+// dual P and the duals Lf, Up of its left and upper neighbours
+__device__ __forceinline__ void rof_primal(const Dual& P, const Dual& Lf,
+                                           const Dual& Up, const float4& fi,
+                                           const float4& ui, const float4& qi,
+                                           const float4& lambda, float4* uo,
+                                           float4* qo) {
   float un[4], qn[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const float lam = comp(lambda, k);
     const float div = ((P.lo[k] - Lf.lo[k]) + P.hi[k]) - Up.hi[k];
-    const float img = comp(fi, k), qq = comp(qi, k), uo = comp(ui, k);
-    float v = (uo + kRofTau * ((lam * img - div) - qq)) /
+    const float img = comp(fi, k), qq = comp(qi, k), u0 = comp(ui, k);
+    float v = (u0 + kRofTau * ((lam * img - div) - qq)) /
               (1.0f + kRofTau * lam);
-    v = v + kRofTheta * (v - uo);
+    v = v + kRofTheta * (v - u0);
     float w = (qq + kRofEta * (v - img)) / (1.0f + 4.0f * kRofEta / lam);
     w = w + kRofTheta * (w - qq);
     un[k] = v;
     qn[k] = w;
   }
-  u_out[o] = mk4(un[0], un[1], un[2], un[3]);
-  q_out[o] = mk4(qn[0], qn[1], qn[2], qn[3]);
-  p_out[2 * o] = mk4(P.lo[0], P.lo[1], P.lo[2], P.lo[3]);
-  p_out[2 * o + 1] = mk4(P.hi[0], P.hi[1], P.hi[2], P.hi[3]);
+  *uo = mk4(un[0], un[1], un[2], un[3]);
+  *qo = mk4(qn[0], qn[1], qn[2], qn[3]);
+}
+
+// Up to kRofK PDDROFDiffTensor steps (PDDROFDiffTensor.cl:43-125) in one
+// launch (temporal tiling).  The reference runs one step per launch over
+// the whole image; a block here owns a kRofTX x kRofTY output tile and
+// works on the region around it, kRofK pixels wider on every side: one
+// wavefront row of 64 columns by kRofRH rows.  Each step forms the duals
+// on the shrinking area that the later steps still need (a pixel's dual
+// from u at the pixel and at its right and lower neighbours), then the
+// primal and q there (from the pixel's dual and those of its left and
+// upper neighbours); after nsteps <= kRofK steps the tile holds what
+// nsteps whole-image steps leave.  The dual stored is the tensor-applied
+// one, as the reference stores it.  u and the duals sit in LDS (the steps
+// read them at neighbours); q, f and the tensor of a thread's own pixels
+// stay in registers.  The halo pixels repeat work with the same
+// operations, so every value is the one a whole-image step forms.
+constexpr int kRofK = kRofStepsMax;
+constexpr int kRofRW = 64;
+constexpr int kRofTX = kRofRW - 2 * kRofK;
+constexpr int kRofTY = 8;
+constexpr int kRofRH = kRofTY + 2 * kRofK;
+constexpr int kRofRows = kRofRH / 4;  // region rows per wavefront
+static_assert(kRofRH % 4 == 0, "four wavefronts share the region rows");
+
+__global__ void __launch_bounds__(256)
+k_rof_steps(int W, int H, int nsteps, const float4* __restrict__ u,
+            const float4* __restrict__ p, const float4* __restrict__ q,
+            const float4* __restrict__ f, const float4* __restrict__ dt,
+            float4 lambda, float4* __restrict__ u_out,
+            float4* __restrict__ p_out, float4* __restrict__ q_out) {
+  __shared__ float4 su[kRofRH * kRofRW];
+  __shared__ float4 sp[2 * kRofRH * kRofRW];
+  const int lx = threadIdx.x;
+  const int rx0 = blockIdx.x * kRofTX - kRofK;
+  const int ry0 = blockIdx.y * kRofTY - kRofK;
+  const int gx = rx0 + lx;
+  const float4 zero = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 fr[kRofRows], dr[kRofRows], qr[kRofRows];
+#pragma unroll
+  for (int r = 0; r < kRofRows; ++r) {
+    const int ly = threadIdx.y * kRofRows + r, gy = ry0 + ly;
+    const int l = ly * kRofRW + lx;
+    fr[r] = dr[r] = qr[r] = zero;
+    if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+      const size_t o = (size_t)gy * W + gx;
+      su[l] = u[o];
+      sp[2 * l] = p[2 * o];
+      sp[2 * l + 1] = p[2 * o + 1];
+      qr[r] = q[o];
+      fr[r] = f[o];
+      dr[r] = dt[o];
+    }
+  }
+  __syncthreads();
+  for (int s = 1; s <= nsteps; ++s) {
+    // duals on [s-1, kRofRW-s) x [s-1, kRofRH-s): the primal area of this
+    // step and its left and upper neighbours
+#pragma unroll
+    for (int r = 0; r < kRofRows; ++r) {
+      const int ly = threadIdx.y * kRofRows + r, gy = ry0 + ly;
+      if (lx < s - 1 || lx >= kRofRW - s || ly < s - 1 || ly >= kRofRH - s)
+        continue;
+      const bool inside = gx >= 0 && gx < W && gy >= 0 && gy < H;
+      // u at the edge-clamped pixel and neighbours, all inside the region
+      const int ci = clampi(gx, 0, W - 1) - rx0;
+      const int ci1 = clampi(gx + 1, 0, W - 1) - rx0;
+      const int cj = (clampi(gy, 0, H - 1) - ry0) * kRofRW;
+      const int cj1 = (clampi(gy + 1, 0, H - 1) - ry0) * kRofRW;
+      const int l = ly * kRofRW + lx;
+      const Dual P =
+          dual_of(su[cj + ci], su[cj + ci1], su[cj1 + ci], dr[r],
+                  inside ? sp[2 * l] : zero, inside ? sp[2 * l + 1] : zero);
+      sp[2 * l] = mk4(P.lo[0], P.lo[1], P.lo[2], P.lo[3]);
+      sp[2 * l + 1] = mk4(P.hi[0], P.hi[1], P.hi[2], P.hi[3]);
+    }
+    __syncthreads();
+    // primal and q on [s, kRofRW-s) x [s, kRofRH-s), pixels in the image
+#pragma unroll
+    for (int r = 0; r < kRofRows; ++r) {
+      const int ly = threadIdx.y * kRofRows + r, gy = ry0 + ly;
+      if (lx < s || lx >= kRofRW - s || ly < s || ly >= kRofRH - s) continue;
+      if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+      const int l = ly * kRofRW + lx;
+      Dual P, Lf, Up;
+      const float4 a = sp[2 * l], b = sp[2 * l + 1];
+      const float4 c = sp[2 * (l - 1)], d = sp[2 * (l - kRofRW) + 1];
+      P.lo[0] = a.x; P.lo[1] = a.y; P.lo[2] = a.z; P.lo[3] = a.w;
+      P.hi[0] = b.x; P.hi[1] = b.y; P.hi[2] = b.z; P.hi[3] = b.w;
+      Lf.lo[0] = c.x; Lf.lo[1] = c.y; Lf.lo[2] = c.z; Lf.lo[3] = c.w;
+      Up.hi[0] = d.x; Up.hi[1] = d.y; Up.hi[2] = d.z; Up.hi[3] = d.w;
+      float4 un, qn;
+      rof_primal(P, Lf, Up, fr[r], su[l], qr[r], lambda, &un, &qn);
+      su[l] = un;
+      qr[r] = qn;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < kRofRows; ++r) {
+    const int ly = threadIdx.y * kRofRows + r, gy = ry0 + ly;
+    if (lx < kRofK || lx >= kRofK + kRofTX || ly < kRofK ||
+        ly >= kRofK + kRofTY)
+      continue;
+    if (gx >= W || gy >= H) continue;
+    const int l = ly * kRofRW + lx;
+    const size_t o = (size_t)gy * W + gx;
+    u_out[o] = su[l];
+    q_out[o] = qr[r];
+    p_out[2 * o] = sp[2 * l];
+    p_out[2 * o + 1] = sp[2 * l + 1];
+  }
 }
 
 // n float4 set to v (graph-safe replacement of a memset)
@@ -187,13 +272,15 @@ hipError_t launch_normal_depth(hipStream_t st, const Geometry& g,
             st, g.W, g.H, states, dispscale, lr, out);
 }
 
-hipError_t launch_rof_step(hipStream_t st, const Geometry& g, const float4* u,
-                           const float4* p, const float4* q, const float4* f,
-                           const float4* dt, float4 lambda, float4* u_out,
-                           float4* p_out, float4* q_out) {
-  PM_LAUNCH(k_rof_step,
-            dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY), kBlock2D, 0,
-            st, g.W, g.H, u, p, q, f, dt, lambda, u_out, p_out, q_out);
+hipError_t launch_rof_steps(hipStream_t st, const Geometry& g, int nsteps,
+                            const float4* u, const float4* p, const float4* q,
+                            const float4* f, const float4* dt, float4 lambda,
+                            float4* u_out, float4* p_out, float4* q_out) {
+  if (nsteps < 1 || nsteps > kRofK) return hipErrorInvalidValue;
+  PM_LAUNCH(k_rof_steps,
+            dim3((g.W + kRofTX - 1) / kRofTX, (g.H + kRofTY - 1) / kRofTY),
+            dim3(kRofRW, 4), 0, st, g.W, g.H, nsteps, u, p, q, f, dt, lambda,
+            u_out, p_out, q_out);
 }
 
 hipError_t launch_fill4(hipStream_t st, size_t n, float4 v, float4* out) {

@@ -263,6 +263,15 @@ class Context:
                                      int(iters), _ptr(out)))
         return out
 
+    def rof_device(self, view, d_f, lam, iters, d_out):
+        """rof() on device pointers (ints) d_f and d_out, W x H x 4 each;
+        enqueued on the context's stream, no synchronisation."""
+        lam = np.ascontiguousarray(lam, np.float32)
+        assert lam.shape == (4,)
+        self._check(self._lib.pm_rof_device(
+            self._h, int(view), ctypes.c_void_p(d_f), _ptr(lam), int(iters),
+            ctypes.c_void_p(d_out)))
+
 
 class PMHuberStereo:
     """cvt::PMHuberStereo (cvt/vision/PMHuberStereo.h:42-110).  By default

@@ -201,6 +201,11 @@ int pm_get_tensor(pm_ctx* ctx, int view, float* out);
  * denoiseDiffusionTensorROF_PDD runs them; out W x H x 4 (host) */
 int pm_rof(pm_ctx* ctx, int view, const float* f, const float* lambda,
            int iters, float* out);
+/* pm_rof on device buffers d_f and d_out (W x H x 4 in HBM, not
+ * overlapping; lambda[4] on the host), stream-ordered: returns after
+ * enqueueing on the context's stream */
+int pm_rof_device(pm_ctx* ctx, int view, const float* d_f,
+                  const float* lambda, int iters, float* d_out);
 /* device evaluation of the engine's scalar math for n inputs:
  * out[5*i .. 5*i+4] = exp, sin, cos, acospi, sqrt of in[i] */
 int pm_math_selftest(pm_ctx* ctx, const float* in, int n, float* out);

@@ -133,3 +133,27 @@ def test_plain_context_has_no_smooth_field(pair):
         ctx.setup()
         with pytest.raises(cvt_amd.PMError):
             ctx.smooth(0)
+
+
+def test_rof_device_matches_host(pair):
+    """pm_rof_device on HBM buffers gives what pm_rof gives; overlapping
+    buffers are refused."""
+    import torch
+    L, R, _ = pair
+    H, W = L.shape[:2]
+    f = np.random.default_rng(5).normal(0, 1, (H, W, 4)).astype(np.float32)
+    lam = F([1.2, 1.2, 1.2, 1.0])
+    print("rof device", flush=True)
+    with huber_ctx(3, 20.0) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        want = ctx.rof(0, f, lam, 7)
+        df = torch.from_numpy(f).to("cuda:0")
+        dout = torch.empty_like(df)
+        torch.cuda.synchronize()
+        ctx.rof_device(0, df.data_ptr(), lam, 7, dout.data_ptr())
+        ctx.sync()
+        got = dout.cpu().numpy()
+        with pytest.raises(cvt_amd.PMError):
+            ctx.rof_device(0, df.data_ptr(), lam, 7, df.data_ptr())
+    assert_same(got, want, "rof_device")
