@@ -170,7 +170,9 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
 // Linear sample (CLK_FILTER_LINEAR, CLK_ADDRESS_CLAMP_TO_EDGE,
 // pmhstereo.cl:71) of the other view at column u of a texel row:
 // t0 + a*(t1 - t0), from adjacent records of the colour and the gradient
-// plane.
+// plane.  The edge-padded rows stand in for the coordinate clamp: every
+// column the kernels sample at lies in [-1, W], and the records it reads
+// there are the image's or copies of its edge texels.
 struct Sample {
   float r, g, b;
   float4 grad;
@@ -179,16 +181,6 @@ struct Sample {
 struct Texel {
   float4 A, G;  // (r, g, b, bilw), gradient
 };
-
-// texel i of a row: colour record from the colour plane, gradient record
-// from the gradient plane gp records further
-__device__ __forceinline__ Texel texel(const float4* __restrict__ srow,
-                                       uint32_t i, size_t gp) {
-  Texel t;
-  t.A = srow[i];
-  t.G = srow[gp + i];
-  return t;
-}
 
 __device__ __forceinline__ Sample lerp_texels(const Texel& t0,
                                               const Texel& t1, float a) {
@@ -201,30 +193,6 @@ __device__ __forceinline__ Sample lerp_texels(const Texel& t0,
   s.grad.z = fmaf(a, t1.G.z - t0.G.z, t0.G.z);
   s.grad.w = fmaf(a, t1.G.w - t0.G.w, t0.G.w);
   return s;
-}
-
-// u clamped to [-1, W] like the sampler's coordinate, then the texel pair
-struct Coord {
-  float a;
-  int f;  // floor of the clamped coordinate, in [-1, W]
-};
-
-__device__ __forceinline__ Coord coord_of(const Lane& L, float u) {
-  u = fmaxf(fminf(u, L.Wf), -1.0f);
-  const float fl = floorf(u);
-  Coord c;
-  c.a = u - fl;
-  c.f = (int)fl;
-  return c;
-}
-
-__device__ __forceinline__ Sample sample(const Lane& L,
-                                         const float4* __restrict__ srow,
-                                         float u) {
-  const Coord c = coord_of(L, u);
-  const uint32_t i0 = (uint32_t)clampi(c.f, 0, L.W - 1);
-  const uint32_t i1 = (uint32_t)clampi(c.f + 1, 0, L.W - 1);
-  return lerp_texels(texel(srow, i0, L.GP), texel(srow, i1, L.GP), c.a);
 }
 
 // texel at record p of the colour plane and p + gp of the gradient plane
@@ -354,8 +322,8 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
       const float rowb = s.y * py;  // transform(), b y per row
       const float4* __restrict__ srow = L.tex2 + (size_t)qy * L.TS;
       const float4* trow = L.trow0 + (dy + R) * L.TW + R;
-      // texel window srow[kf - 1 .. kf + 2] of the previous fast-path tap;
-      // a slope near 1 slides it by one texel per tap
+      // texel window srow[kf - 1 .. kf + 2] of the previous tap; a slope
+      // near 1 slides it by one texel per tap
       int kf = INT_MIN;
       Texel tw0, tw1, tw2, tw3;
       float px = L.xf + (float)dx0;  // stepped by 1, exact below 2^24
@@ -367,46 +335,42 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
         const float w1 = support_weight(v0, v1, v2, L.cw);
         const float k1 = (w1 * 0.125f) * 0.33f, k2 = w1 * 0.875f;
         const float xq = fmaf(s.x, px, rowb) + s.z;
-        Sample sm, s0, sp;
         // The samples at x'-1, x' and x'+1 read the four texels from
-        // floor(xc)-1 on, xc = x' clamped to [-1, W]:
+        // floor(xc)-1 on, xc = x' clamped to [-1, W] (a NaN x' clamps to W):
         //  - x'-1 is exact for x' >= 0.5 and shares the floor and fraction
         //    of x'; below that it and its neighbour both read texel 0 (the
         //    left pads), whatever the fraction;
         //  - past either edge every texel involved is a copy of the edge
         //    texel, which is what the clamped sampler returns;
         //  - x'+1 is rounded and has its own fraction; its floor is
-        //    floor(xc)+1 unless the sum rounded up to the next integer,
-        //    which takes the general path.
+        //    floor(xc)+1, or floor(xc)+2 when the sum rounded up to the
+        //    next integer (xc < W, so texel floor(xc)+3 is at most the
+        //    last right pad).
         const float xc = fmaxf(fminf(xq, L.Wf), -1.0f);
         const float xp1 = xc + 1.0f;
         const float fl = floorf(xc);
         const float flp = floorf(xp1);
-        if (__all(flp == fl + 1.0f)) {
-          const float a = xc - fl;
-          const float ap = xp1 - flp;
-          const int f = (int)fl;  // -1 <= f <= W
-          if (f != kf) {
-            if (f == kf + 1) {
-              tw0 = tw1;
-              tw1 = tw2;
-              tw2 = tw3;
-            } else {
-              tw0 = texel_at(srow + (f - 1), L.GP);
-              tw1 = texel_at(srow + f, L.GP);
-              tw2 = texel_at(srow + (f + 1), L.GP);
-            }
-            tw3 = texel_at(srow + (f + 2), L.GP);
-            kf = f;
+        const float a = xc - fl;
+        const float ap = xp1 - flp;
+        const int f = (int)fl;  // -1 <= f <= W
+        if (f != kf) {
+          if (f == kf + 1) {
+            tw0 = tw1;
+            tw1 = tw2;
+            tw2 = tw3;
+          } else {
+            tw0 = texel_at(srow + (f - 1), L.GP);
+            tw1 = texel_at(srow + f, L.GP);
+            tw2 = texel_at(srow + (f + 1), L.GP);
           }
-          sm = lerp_texels(tw0, tw1, a);
-          s0 = lerp_texels(tw1, tw2, a);
-          sp = lerp_texels(tw2, tw3, ap);
-        } else {
-          sm = sample(L, srow, xq - 1.0f);
-          s0 = sample(L, srow, xq);
-          sp = sample(L, srow, xp1);
+          tw3 = texel_at(srow + (f + 2), L.GP);
+          kf = f;
         }
+        const Sample sm = lerp_texels(tw0, tw1, a);
+        const Sample s0 = lerp_texels(tw1, tw2, a);
+        Sample sp = lerp_texels(tw2, tw3, ap);
+        if (flp != fl + 1.0f)
+          sp = lerp_texels(tw3, texel_at(srow + (f + 3), L.GP), ap);
         const float C = klt_term(s0, v0, v1, v2, ga, k1, k2);
         const float Cxp = klt_term(sp, v0, v1, v2, ga, k1, k2);
         const float Cxm = klt_term(sm, v0, v1, v2, ga, k1, k2);

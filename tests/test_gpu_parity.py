@@ -305,3 +305,71 @@ def test_error_paths():
         assert e.value.code == 1
         with pytest.raises(cvt_amd.PMError):
             ctx.disparity(pp=7)
+
+
+def _random_pair(W, H, C, seed):
+    """A textured pair whose right view is the left shifted by one column
+    (edge column repeated); random enough that every candidate kind wins
+    somewhere."""
+    rng = np.random.default_rng(seed)
+    shape = (H, W) if C == 1 else (H, W, 4)
+    L = rng.random(shape).astype(np.float32)
+    R = np.concatenate([L[:, 1:], L[:, -1:]], axis=1) if W > 1 else L.copy()
+    R = np.ascontiguousarray(R + np.float32(0.01) *
+                             rng.standard_normal(shape).astype(np.float32))
+    return L, R
+
+
+@pytest.mark.parametrize("W,H,C,radius", [
+    (1, 1, 4, 2),     # one pixel: no KLT taps, singular KLT system
+    (1, 9, 1, 3),     # one column
+    (11, 1, 4, 2),    # one row, narrower than a wavefront
+    (5, 3, 1, 4),     # image smaller than the patch
+    (70, 6, 4, 1),    # smallest patch, a partial second block column
+    (40, 21, 1, 16),  # the largest patch the engine accepts
+])
+def test_small_and_odd_shapes_bitexact(oracle, W, H, C, radius):
+    """Edges everywhere: patches, KLT windows, texel pads and tiles reach
+    past the image on every side; planes of every iteration, then the
+    LR-checked disparity, bit for bit."""
+    L, R = _random_pair(W, H, C, seed=W * 100 + H)
+    dmax, iters = float(max(2, W // 2)), 2
+    print("shape", W, H, C, radius, flush=True)
+    want = run_oracle_iterations(oracle, L, R, radius, dmax, iters)
+    ref = oracle.run(L, R, radius, dmax, iters, maxdisp=1.0, maxdeg=180.0)
+    with cvt_amd.Context(radius=radius, disparity_max=dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for it in range(iters):
+            ctx.iterate(1)
+            for view in (0, 1):
+                got = ctx.planes(view)
+                assert same(got, want[it][view]), (
+                    it, view, first_diff(got, want[it][view]))
+        d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 180.0)
+    assert same(d, ref["disp"]), first_diff(d, ref["disp"])
+
+
+@pytest.mark.parametrize("kind", ["flat", "saturated"])
+def test_flat_and_saturated_images_bitexact(oracle, kind):
+    """A flat image (every support weight 1, every plane costs the same)
+    and one whose colour steps make nearly every off-centre support weight
+    underflow to 0 (most evaluations end at the wsum <= 1 rule)."""
+    W, H = 90, 13
+    if kind == "flat":
+        L = np.full((H, W, 4), 0.5, np.float32)
+        R = L.copy()
+    else:
+        rng = np.random.default_rng(3)
+        L = (rng.random((H, W, 4)) * 400.0).astype(np.float32)
+        R = np.ascontiguousarray(np.roll(L, 2, axis=1))
+    print("image", kind, flush=True)
+    want = run_oracle_iterations(oracle, L, R, 3, 12.0, 2)
+    with cvt_amd.Context(radius=3, disparity_max=12.0) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(2)
+        for view in (0, 1):
+            got = ctx.planes(view)
+            assert same(got, want[1][view]), (view,
+                                               first_diff(got, want[1][view]))
