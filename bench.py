@@ -36,9 +36,10 @@ FLOP_PER_COST_TAP = 50
 FLOP_PER_KLT_TAP = 150
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per k_propagate launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# runs, scaled by the same counters read for a copy kernel of known size;
-# used only while the kernel sources still hash to what was profiled
+# HBM bytes per view propagate (its two k_propagate half launches) from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by the same counters read
+# for a copy kernel of known size; used only while the kernel sources
+# still hash to what was profiled
 HBM_PROFILE = os.path.join(ROOT, "profiles",
                            "k_propagate_1280x720_rgba_r4_hbm.json")
 KERNEL_SOURCES = ("cvt_amd/csrc/pm_engine.hip", "cvt_amd/csrc/pm_device.h")
@@ -49,8 +50,8 @@ def log(*a):
 
 
 def profiled_traffic():
-    """Calibrated HBM bytes per launch of the profiled kernel, or None when
-    the kernel sources changed since the counters were collected."""
+    """Calibrated HBM bytes per view propagate, or None when the kernel
+    sources changed since the counters were collected."""
     import hashlib
     try:
         with open(HBM_PROFILE) as f:
@@ -63,7 +64,7 @@ def profiled_traffic():
         return None
     if h.hexdigest() != prof.get("source_sha256"):
         return None
-    return float(prof["hbm_bytes_per_dispatch"])
+    return float(prof["hbm_bytes_per_propagate"])
 
 
 def cpu_baseline(L, R, gt):
@@ -239,12 +240,13 @@ def main():
     def step(events=None):
         ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
         ctx.setup()
-        for i in range(ITERS):
-            if events is not None:
-                events[i][0].record(stream)
-            ctx.iterate(1)
-            if events is not None:
-                events[i][1].record(stream)
+        # the iterations as one call, as disparityMap runs them: the engine
+        # then overlaps consecutive propagates (two half launches each)
+        if events is not None:
+            events[0].record(stream)
+        ctx.iterate(ITERS)
+        if events is not None:
+            events[1].record(stream)
         ctx.disparity_device(dD.data_ptr(), cvt_amd.DISPARITY_PP_LRCHECK,
                              MAXDISP, MAXDEG)
 
@@ -252,8 +254,8 @@ def main():
         for i in range(args.warmup):
             step()
             log("bench: warmup step %d done" % (i + 1))
-        ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)]
-               for _ in range(ITERS)] for _ in range(args.steps)]
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)]
+              for _ in range(args.steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(args.steps):
@@ -262,10 +264,11 @@ def main():
         t1 = time.perf_counter()
         log("bench: %d timed steps done" % args.steps)
         disp = dD.cpu().numpy()
-        # per iteration = one k_propagate per view (+ two 8 B/pixel buffer
-        # clears, < 0.1 %)
-        iter_ms = [ev[k][i][0].elapsed_time(ev[k][i][1])
-                   for k in range(args.steps) for i in range(ITERS)]
+        # per iteration = one propagate per view, each two k_propagate
+        # half launches on two streams (+ the 8 B/pixel slot clears, <
+        # 0.1 %); a propagate's time is its share of the iterations' span
+        iter_ms = [ev[k][0].elapsed_time(ev[k][1]) / ITERS
+                   for k in range(args.steps)]
         prop_ms = float(np.mean(iter_ms)) / 2.0
         # the same pair from and to host memory (pm_set_images /
         # pm_disparity): the PCIe-inclusive rate, reported beside `value`
@@ -285,9 +288,9 @@ def main():
             rof_ms = time_rof_launch(ctx, stream)
     ctx.close()
 
-    launches = 2 * ITERS
+    propagates = 2 * ITERS
     flop = (FLOP_PER_COST_TAP * cost_taps + FLOP_PER_KLT_TAP * klt_taps) \
-        / launches
+        / propagates
     achieved = flop / (prop_ms * 1e-3) / 1e12
     elapsed = t1 - t0
     value = args.steps / elapsed
@@ -321,10 +324,11 @@ def main():
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": profiled_traffic(),
-                     "traffic_unit": "bytes per launch",
-                     "avg_launch_ms": prop_ms,
-                     "cost_taps_per_launch": cost_taps / launches,
-                     "klt_taps_per_launch": klt_taps / launches},
+                     "traffic_unit": "bytes per view propagate",
+                     "avg_propagate_ms": prop_ms,
+                     "launches_per_propagate": 2,
+                     "cost_taps_per_propagate": cost_taps / propagates,
+                     "klt_taps_per_propagate": klt_taps / propagates},
     }
     if huber:
         # the PM-Huber pipeline: same workload, smoothing after every

@@ -63,6 +63,13 @@ struct pm_ctx {
   DevBuf rof_q[2];      // ROF duals q, ping-pong
   float theta = 0.0f;   // theta of the next iteration
   int schedule_n = 1;   // N of the theta schedule of the running pm_iterate
+
+  // Split schedule of the iterations: each view's propagate runs as a top
+  // and a bottom half, the top halves on `stream`, the bottom halves on
+  // `side`, so that one half's last work-groups overlap the next half's
+  // first.  ev[k] marks the end of the latest launch of half k.
+  hipStream_t side = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -319,7 +326,75 @@ int smooth_view(pm_ctx* c, int view, const float4* planes, float theta) {
              f4buf(c->smooth[view]));
 }
 
-int one_iteration(pm_ctx* c, unsigned long long* counters = nullptr) {
+// Halves of the split schedule (ctx.ev indices)
+enum { kTopL = 0, kBotL = 1, kTopR = 2, kBotR = 3 };
+
+// block rows of the top half; 0 when the image is too short to split
+int split_rows(const pm_ctx* c) {
+  const int nb = (c->geo.H + pm::kBlockRows - 1) / pm::kBlockRows;
+  return nb < 2 ? 0 : nb / 2;
+}
+
+// Makes everything enqueued on the context's stream so far a predecessor
+// of the next half launched on either stream (all four events recorded
+// on the context's stream).
+int split_rebase(pm_ctx* c) {
+  if (!c->side) {
+    PM_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    for (hipEvent_t& e : c->ev)
+      PM_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  for (hipEvent_t e : c->ev) PM_HIP(c, hipEventRecord(e, c->stream));
+  return PM_OK;
+}
+
+// One view's propagate as two halves.  Each half reads, besides its own
+// rows, the view's previous field within 5 rows of them (the random
+// neighbours) and writes its rows of the new field and of the scatter
+// slots; the other view's field and slots it reads are on its own rows.
+// So a half waits for the other half of the same view's previous
+// propagate (rows it reads, and rows of the buffer it overwrites that
+// that half still reads) and follows its own stream otherwise: the top
+// half of the right view follows the top half of the left view, and so
+// on.  Both waits are enqueued before this propagate's events are
+// recorded, so they refer to the previous propagate of the view.
+int propagate_split(pm_ctx* c, const pm::PropagateArgs& a, int view,
+                    unsigned long long* vb, unsigned long long* counters) {
+  const int nbt = split_rows(c);
+  const int rows_top = nbt * pm::kBlockRows;
+  pm::Geometry top = c->geo, bot = c->geo;
+  top.by0 = 0;
+  top.nby = nbt;
+  bot.by0 = nbt;
+  bot.nby = (c->geo.H + pm::kBlockRows - 1) / pm::kBlockRows - nbt;
+  hipEvent_t& ev_top = c->ev[view == 0 ? kTopL : kTopR];
+  hipEvent_t& ev_bot = c->ev[view == 0 ? kBotL : kBotR];
+  PM_HIP(c, hipStreamWaitEvent(c->side, ev_top, 0));
+  PM_HIP(c, hipStreamWaitEvent(c->stream, ev_bot, 0));
+  PM_HIP(c, pm::launch_viewbuf_clear_rows(c->stream, c->geo, vb, 0, rows_top));
+  PM_HIP(c, pm::launch_propagate(c->stream, top, a, counters));
+  PM_HIP(c, hipEventRecord(ev_top, c->stream));
+  PM_HIP(c, pm::launch_viewbuf_clear_rows(c->side, c->geo, vb, rows_top,
+                                          c->geo.H - rows_top));
+  PM_HIP(c, pm::launch_propagate(c->side, bot, a, counters));
+  PM_HIP(c, hipEventRecord(ev_bot, c->side));
+  return PM_OK;
+}
+
+// One view's propagate: the whole image on the context's stream, or as
+// two halves when the caller set up the split schedule (split_rebase).
+int propagate_view(pm_ctx* c, const pm::PropagateArgs& a, int view,
+                   unsigned long long* vb, unsigned long long* counters,
+                   bool split) {
+  if (split) return propagate_split(c, a, view, vb, counters);
+  PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vb));
+  PM_HIP(c, pm::launch_propagate(c->stream, c->geo, a, counters));
+  return PM_OK;
+}
+
+// pmhstereo_propagate_view for the left then the right view, then the
+// PM-Huber smoothing (PMHuberStereo.cpp:530-578)
+int one_iteration(pm_ctx* c, unsigned long long* counters, bool split) {
   const int cur = c->cur, nxt = 1 - c->cur;
   unsigned long long* vbL = static_cast<unsigned long long*>(c->vbuf[0].ptr);
   unsigned long long* vbR = static_cast<unsigned long long*>(c->vbuf[1].ptr);
@@ -329,8 +404,8 @@ int one_iteration(pm_ctx* c, unsigned long long* counters = nullptr) {
   a.dmax = c->prm.disparity_max;
   a.theta = huber ? theta * kThetaScale : 0.0f;
   a.iter = c->iteration;
+  int rc;
 
-  PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vbL));
   a.self = view_bufs(c, 0);
   a.other = view_bufs(c, 1);
   a.old = state_ptr(c, 0, cur);
@@ -340,9 +415,8 @@ int one_iteration(pm_ctx* c, unsigned long long* counters = nullptr) {
   a.viewout = vbL;
   a.out = state_ptr(c, 0, nxt);
   a.lr = 1;
-  PM_HIP(c, pm::launch_propagate(c->stream, c->geo, a, counters));
+  if ((rc = propagate_view(c, a, 0, vbL, counters, split))) return rc;
 
-  PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vbR));
   a.self = view_bufs(c, 1);
   a.other = view_bufs(c, 0);
   a.old = state_ptr(c, 1, cur);
@@ -352,12 +426,15 @@ int one_iteration(pm_ctx* c, unsigned long long* counters = nullptr) {
   a.viewout = vbR;
   a.out = state_ptr(c, 1, nxt);
   a.lr = 0;
-  PM_HIP(c, pm::launch_propagate(c->stream, c->geo, a, counters));
+  if ((rc = propagate_view(c, a, 1, vbR, counters, split))) return rc;
 
-  int rc;
   if (huber) {
+    // the smoothing reads both halves of both views; the next iteration's
+    // halves read the smooth fields
+    if (split) PM_HIP(c, hipStreamWaitEvent(c->stream, c->ev[kBotR], 0));
     if ((rc = smooth_view(c, 0, state_ptr(c, 0, nxt), theta))) return rc;
     if ((rc = smooth_view(c, 1, state_ptr(c, 1, nxt), theta))) return rc;
+    if (split && (rc = split_rebase(c))) return rc;
     // theta = (iter / N)^4 + theta_min (PMHuberStereo.cpp:575-576)
     c->theta = std::pow((float)c->iteration / (float)c->schedule_n, 4.0f) +
                c->prm.theta_min;
@@ -365,6 +442,24 @@ int one_iteration(pm_ctx* c, unsigned long long* counters = nullptr) {
   c->cur = nxt;
   c->iteration += 1;
   return PM_OK;
+}
+
+// n iterations; with the split schedule the context's stream finally
+// waits for the last bottom half, so later work on it sees every result
+int iterate_n(pm_ctx* c, int n, unsigned long long* counters) {
+  const bool split = n > 0 && split_rows(c) > 0;
+  int rc;
+  if (split && (rc = split_rebase(c))) return rc;
+  for (int i = 0; i < n; ++i)
+    if ((rc = one_iteration(c, counters, split))) break;
+  if (split) {
+    // joined even after a failed launch, so that no work is left behind
+    // unordered on the side stream
+    const hipError_t e = hipStreamWaitEvent(c->stream, c->ev[kBotR], 0);
+    if (!rc && e != hipSuccess)
+      return hip_fail(c, e, "hipStreamWaitEvent");
+  }
+  return rc;
 }
 
 // N of the theta schedule for a pm_iterate call of n iterations
@@ -490,6 +585,12 @@ void pm_destroy(pm_ctx* c) {
   release(c->counters);
   release(c->nd);
   release(c->rof_u);
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
+  for (hipEvent_t e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -575,8 +676,7 @@ int pm_iterate(pm_ctx* c, int n) {
   if ((rc = require_ready(c)) || (rc = use_device(c))) return rc;
   if (n < 0) return fail(c, PM_ERR_ARG, "negative iteration count %d", n);
   start_schedule(c, n);
-  for (int i = 0; i < n; ++i)
-    if ((rc = one_iteration(c))) return rc;
+  if ((rc = iterate_n(c, n, nullptr))) return rc;
   c->last_pp = -1;
   return PM_OK;
 }
@@ -595,8 +695,7 @@ int pm_iterate_counted(pm_ctx* c, int n, uint64_t* cost_taps,
   const unsigned long long zero[2] = {0ull, 0ull};
   if ((rc = to_device(c, d, zero, sizeof zero))) return rc;
   start_schedule(c, n);
-  for (int i = 0; i < n; ++i)
-    if ((rc = one_iteration(c, d))) return rc;
+  if ((rc = iterate_n(c, n, d))) return rc;
   unsigned long long h[2];
   if ((rc = to_host(c, h, d, sizeof h))) return rc;
   *cost_taps = h[0];

@@ -31,10 +31,18 @@ __host__ __device__ inline int tex_stride(int W) {
   return W + kTexPadL + kTexPadR;
 }
 
+// work-group tiles of the cost-evaluating kernels: kBlockCols x kBlockRows
+// pixels
+constexpr int kBlockCols = 64;
+constexpr int kBlockRows = 4;
+
 struct Geometry {
   int W = 0, H = 0, R = 0;
   int TS = 0;  // texel row stride in records, tex_stride(W)
   size_t npix = 0;
+  // a launch covers the block rows [by0, by0 + nby) of the image
+  // (nby = 0: all of them)
+  int by0 = 0, nby = 0;
   float color_weight = 12.0f;
 };
 
@@ -52,6 +60,10 @@ hipError_t launch_init_prior(hipStream_t st, const Geometry& g, ViewBufs self,
                              const float* disp, const float4* normals,
                              float4* states);
 
+// clear the view-scatter slots of image rows [row0, row0 + rows)
+hipError_t launch_viewbuf_clear_rows(hipStream_t st, const Geometry& g,
+                                     unsigned long long* vbuf, int row0,
+                                     int rows);
 hipError_t launch_viewbuf_clear(hipStream_t st, const Geometry& g,
                                 unsigned long long* vbuf);
 

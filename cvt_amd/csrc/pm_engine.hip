@@ -67,12 +67,14 @@ __device__ __forceinline__ Mwc mwc_jump(uint64_t k) {
 // ---------------------------------------------------------------------
 // block geometry and LDS layout
 // ---------------------------------------------------------------------
-constexpr int kTX = 64;  // block width = one wavefront
-constexpr int kTY = 4;   // wavefronts (rows) per block
+constexpr int kTX = kBlockCols;  // block width = one wavefront
+constexpr int kTY = kBlockRows;  // wavefronts (rows) per block
 constexpr int kXcds = 8;
 
-__host__ __forceinline__ dim3 grid_for(int W, int H) {
-  return dim3((unsigned)((W + kTX - 1) / kTX), (unsigned)((H + kTY - 1) / kTY));
+// the blocks of a launch: every block column, the launch's block rows
+__host__ __forceinline__ dim3 grid_for(const Geometry& g) {
+  const int nby = g.nby > 0 ? g.nby : (g.H + kTY - 1) / kTY;
+  return dim3((unsigned)((g.W + kTX - 1) / kTX), (unsigned)nby);
 }
 
 // LDS bytes of the block kernels: the own-view tile
@@ -126,6 +128,7 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
                             const ViewBufs& other, float4* smem) {
   int bx, by;
   xcd_block(&bx, &by);
+  by += g.by0;
   const int R = g.R, W = g.W, H = g.H;
   const int TW = kTX + 2 * R, TH = kTY + 2 * R;
   const int x0 = bx * kTX, y0 = by * kTY;
@@ -911,7 +914,7 @@ static hipError_t allow_lds(K kernel, size_t bytes) {
     const size_t lds_ = block_lds_bytes((g).R);                         \
     hipError_t e_ = allow_lds(kernel, lds_);                            \
     if (e_ != hipSuccess) return e_;                                    \
-    PM_LAUNCH(kernel, grid_for((g).W, (g).H), kBlockTile, lds_, st,     \
+    PM_LAUNCH(kernel, grid_for(g), kBlockTile, lds_, st,                \
               __VA_ARGS__);                                             \
   } while (0)
 
@@ -940,6 +943,17 @@ hipError_t launch_init_prior(hipStream_t st, const Geometry& g, ViewBufs self,
                              float4* states) {
   PM_LAUNCH_TILE(k_init_prior, g, st, g, self, other, dmax, lr, disp, normals,
                  states);
+}
+
+hipError_t launch_viewbuf_clear_rows(hipStream_t st, const Geometry& g,
+                                     unsigned long long* vbuf, int row0,
+                                     int rows) {
+  if (row0 < 0 || rows < 0 || row0 + rows > g.H) return hipErrorInvalidValue;
+  const size_t n = (size_t)rows * g.W;
+  if (n == 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  PM_LAUNCH(k_viewbuf_clear, dim3(blocks), dim3(256), 0, st, n,
+            vbuf + (size_t)row0 * g.W);
 }
 
 hipError_t launch_viewbuf_clear(hipStream_t st, const Geometry& g,
