@@ -57,19 +57,24 @@ struct pm_ctx {
   // PM-Huber coupling (PM_SMOOTH_HUBER only)
   DevBuf dt[2];         // diffusion tensor per view
   DevBuf smooth[2];     // smoothed (normal, depth) field per view
-  DevBuf nd;            // (normal, depth) field being smoothed
-  DevBuf rof_u;         // ROF primal scratch
-  DevBuf rof_p[2];      // ROF duals p (two float4 per pixel), ping-pong
-  DevBuf rof_q[2];      // ROF duals q, ping-pong
+  // ROF scratch per view (the two views' smoothings may overlap)
+  struct RofBufs {
+    DevBuf nd;          // (normal, depth) field being smoothed
+    DevBuf u;           // primal scratch
+    DevBuf p[2];        // duals p (two float4 per pixel), ping-pong
+    DevBuf q[2];        // duals q, ping-pong
+  } rofb[2];
   float theta = 0.0f;   // theta of the next iteration
   int schedule_n = 1;   // N of the theta schedule of the running pm_iterate
 
   // Split schedule of the iterations: each view's propagate runs as a top
   // and a bottom half, the top halves on `stream`, the bottom halves on
   // `side`, so that one half's last work-groups overlap the next half's
-  // first.  ev[k] marks the end of the latest launch of half k.
-  hipStream_t side = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // first; with PM_SMOOTH_HUBER each view's smoothing runs on `smoothing`
+  // beside the other view's propagate.  ev[k] marks the end of the latest
+  // launch of half k (or of view k - 4's smoothing).
+  hipStream_t side = nullptr, smoothing = nullptr;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -182,11 +187,15 @@ int alloc_pair(pm_ctx* c, int W, int H, int channels) {
     for (int v = 0; v < 2; ++v) {
       if ((rc = ensure(c, c->dt[v], n * sizeof(float4)))) return rc;
       if ((rc = ensure(c, c->smooth[v], n * sizeof(float4)))) return rc;
-      if ((rc = ensure(c, c->rof_p[v], 2 * n * sizeof(float4)))) return rc;
-      if ((rc = ensure(c, c->rof_q[v], n * sizeof(float4)))) return rc;
+      pm_ctx::RofBufs& b = c->rofb[v];
+      if ((rc = ensure(c, b.nd, n * sizeof(float4))) ||
+          (rc = ensure(c, b.u, n * sizeof(float4))))
+        return rc;
+      for (int k = 0; k < 2; ++k)
+        if ((rc = ensure(c, b.p[k], 2 * n * sizeof(float4))) ||
+            (rc = ensure(c, b.q[k], n * sizeof(float4))))
+          return rc;
     }
-    if ((rc = ensure(c, c->nd, n * sizeof(float4)))) return rc;
-    if ((rc = ensure(c, c->rof_u, n * sizeof(float4)))) return rc;
   }
   c->geo.W = W;
   c->geo.H = H;
@@ -287,11 +296,12 @@ float4 huber_lambda(float theta) {
 // when the count is odd, so that the result lands in out).  f must not
 // alias out.
 int rof(pm_ctx* c, int view, const float4* f, float4 lambda, int iters,
-        float4* out) {
+        float4* out, hipStream_t st) {
   const size_t n = c->geo.npix;
-  float4* u[2] = {f4buf(c->rof_u), out};
-  float4* p[2] = {f4buf(c->rof_p[0]), f4buf(c->rof_p[1])};
-  float4* q[2] = {f4buf(c->rof_q[0]), f4buf(c->rof_q[1])};
+  pm_ctx::RofBufs& b = c->rofb[view];
+  float4* u[2] = {f4buf(b.u), out};
+  float4* p[2] = {f4buf(b.p[0]), f4buf(b.p[1])};
+  float4* q[2] = {f4buf(b.q[0]), f4buf(b.q[1])};
   const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float4* dt = f4buf(c->dt[view]);
   // The reference ping-pongs one step per launch and adds a step when the
@@ -303,12 +313,12 @@ int rof(pm_ctx* c, int view, const float4* f, float4 lambda, int iters,
   const int launches = (steps + pm::kRofStepsMax - 1) / pm::kRofStepsMax;
   int src = (launches & 1) ? 0 : 1;
   PM_HIP(c, hipMemcpyAsync(u[src], f, n * sizeof(float4),
-                           hipMemcpyDeviceToDevice, c->stream));
-  PM_HIP(c, pm::launch_fill4(c->stream, 2 * n, zero, p[src]));
-  PM_HIP(c, pm::launch_fill4(c->stream, n, zero, q[src]));
+                           hipMemcpyDeviceToDevice, st));
+  PM_HIP(c, pm::launch_fill4(st, 2 * n, zero, p[src]));
+  PM_HIP(c, pm::launch_fill4(st, n, zero, q[src]));
   for (int done = 0; done < steps; done += pm::kRofStepsMax) {
     const int k = std::min(pm::kRofStepsMax, steps - done);
-    PM_HIP(c, pm::launch_rof_steps(c->stream, c->geo, k, u[src], p[src],
+    PM_HIP(c, pm::launch_rof_steps(st, c->geo, k, u[src], p[src],
                                    q[src], f, dt, lambda, u[1 - src],
                                    p[1 - src], q[1 - src]));
     src = 1 - src;
@@ -318,16 +328,18 @@ int rof(pm_ctx* c, int view, const float4* f, float4 lambda, int iters,
 
 // (normal, depth) field of a view's current planes, then smoothed into the
 // view's smooth field (PMHuberStereo.cpp:552-567)
-int smooth_view(pm_ctx* c, int view, const float4* planes, float theta) {
-  PM_HIP(c, pm::launch_normal_depth(c->stream, c->geo, planes,
+int smooth_view(pm_ctx* c, int view, const float4* planes, float theta,
+                hipStream_t st) {
+  float4* nd = f4buf(c->rofb[view].nd);
+  PM_HIP(c, pm::launch_normal_depth(st, c->geo, planes,
                                     1.0f / c->prm.disparity_max,
-                                    view == 0 ? 1 : 0, f4buf(c->nd)));
-  return rof(c, view, f4buf(c->nd), huber_lambda(theta), kRofIters,
-             f4buf(c->smooth[view]));
+                                    view == 0 ? 1 : 0, nd));
+  return rof(c, view, nd, huber_lambda(theta), kRofIters,
+             f4buf(c->smooth[view]), st);
 }
 
-// Halves of the split schedule (ctx.ev indices)
-enum { kTopL = 0, kBotL = 1, kTopR = 2, kBotR = 3 };
+// Halves and smoothings of the split schedule (ctx.ev indices)
+enum { kTopL = 0, kBotL = 1, kTopR = 2, kBotR = 3, kSmL = 4, kSmR = 5 };
 
 // block rows of the top half; 0 when the image is too short to split
 int split_rows(const pm_ctx* c) {
@@ -341,6 +353,7 @@ int split_rows(const pm_ctx* c) {
 int split_rebase(pm_ctx* c) {
   if (!c->side) {
     PM_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    PM_HIP(c, hipStreamCreateWithFlags(&c->smoothing, hipStreamNonBlocking));
     for (hipEvent_t& e : c->ev)
       PM_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
@@ -371,6 +384,12 @@ int propagate_split(pm_ctx* c, const pm::PropagateArgs& a, int view,
   hipEvent_t& ev_bot = c->ev[view == 0 ? kBotL : kBotR];
   PM_HIP(c, hipStreamWaitEvent(c->side, ev_top, 0));
   PM_HIP(c, hipStreamWaitEvent(c->stream, ev_bot, 0));
+  if (c->prm.smoothing == PM_SMOOTH_HUBER) {
+    // the view's smooth field, from the smoothing after its last propagate
+    hipEvent_t sm = c->ev[view == 0 ? kSmL : kSmR];
+    PM_HIP(c, hipStreamWaitEvent(c->side, sm, 0));
+    PM_HIP(c, hipStreamWaitEvent(c->stream, sm, 0));
+  }
   PM_HIP(c, pm::launch_viewbuf_clear_rows(c->stream, c->geo, vb, 0, rows_top));
   PM_HIP(c, pm::launch_propagate(c->stream, top, a, counters));
   PM_HIP(c, hipEventRecord(ev_top, c->stream));
@@ -392,8 +411,27 @@ int propagate_view(pm_ctx* c, const pm::PropagateArgs& a, int view,
   return PM_OK;
 }
 
-// pmhstereo_propagate_view for the left then the right view, then the
-// PM-Huber smoothing (PMHuberStereo.cpp:530-578)
+// The smoothing of a view's new field (PMHuberStereo.cpp:552-567).  In the
+// split schedule it runs on the smoothing stream once both halves of the
+// view's propagate are done, beside the other view's propagate; the next
+// propagate of the same view, which reads the smooth field, waits for it
+// (propagate_view), and so does every later user of the field buffers it
+// reads, since they follow that propagate.
+int smooth_after(pm_ctx* c, int view, const float4* planes, float theta,
+                 bool split) {
+  if (!split) return smooth_view(c, view, planes, theta, c->stream);
+  PM_HIP(c, hipStreamWaitEvent(c->smoothing,
+                               c->ev[view == 0 ? kTopL : kTopR], 0));
+  PM_HIP(c, hipStreamWaitEvent(c->smoothing,
+                               c->ev[view == 0 ? kBotL : kBotR], 0));
+  int rc;
+  if ((rc = smooth_view(c, view, planes, theta, c->smoothing))) return rc;
+  PM_HIP(c, hipEventRecord(c->ev[view == 0 ? kSmL : kSmR], c->smoothing));
+  return PM_OK;
+}
+
+// pmhstereo_propagate_view for the left then the right view, each followed
+// by its PM-Huber smoothing (PMHuberStereo.cpp:530-578)
 int one_iteration(pm_ctx* c, unsigned long long* counters, bool split) {
   const int cur = c->cur, nxt = 1 - c->cur;
   unsigned long long* vbL = static_cast<unsigned long long*>(c->vbuf[0].ptr);
@@ -416,6 +454,8 @@ int one_iteration(pm_ctx* c, unsigned long long* counters, bool split) {
   a.out = state_ptr(c, 0, nxt);
   a.lr = 1;
   if ((rc = propagate_view(c, a, 0, vbL, counters, split))) return rc;
+  if (huber && (rc = smooth_after(c, 0, state_ptr(c, 0, nxt), theta, split)))
+    return rc;
 
   a.self = view_bufs(c, 1);
   a.other = view_bufs(c, 0);
@@ -427,14 +467,10 @@ int one_iteration(pm_ctx* c, unsigned long long* counters, bool split) {
   a.out = state_ptr(c, 1, nxt);
   a.lr = 0;
   if ((rc = propagate_view(c, a, 1, vbR, counters, split))) return rc;
+  if (huber && (rc = smooth_after(c, 1, state_ptr(c, 1, nxt), theta, split)))
+    return rc;
 
   if (huber) {
-    // the smoothing reads both halves of both views; the next iteration's
-    // halves read the smooth fields
-    if (split) PM_HIP(c, hipStreamWaitEvent(c->stream, c->ev[kBotR], 0));
-    if ((rc = smooth_view(c, 0, state_ptr(c, 0, nxt), theta))) return rc;
-    if ((rc = smooth_view(c, 1, state_ptr(c, 1, nxt), theta))) return rc;
-    if (split && (rc = split_rebase(c))) return rc;
     // theta = (iter / N)^4 + theta_min (PMHuberStereo.cpp:575-576)
     c->theta = std::pow((float)c->iteration / (float)c->schedule_n, 4.0f) +
                c->prm.theta_min;
@@ -454,10 +490,12 @@ int iterate_n(pm_ctx* c, int n, unsigned long long* counters) {
     if ((rc = one_iteration(c, counters, split))) break;
   if (split) {
     // joined even after a failed launch, so that no work is left behind
-    // unordered on the side stream
-    const hipError_t e = hipStreamWaitEvent(c->stream, c->ev[kBotR], 0);
-    if (!rc && e != hipSuccess)
-      return hip_fail(c, e, "hipStreamWaitEvent");
+    // unordered on the other streams: the last bottom half and the last
+    // smoothing (the smoothing stream runs its launches in order)
+    for (int k : {kBotR, kSmR}) {
+      const hipError_t e = hipStreamWaitEvent(c->stream, c->ev[k], 0);
+      if (!rc && e != hipSuccess) rc = hip_fail(c, e, "hipStreamWaitEvent");
+    }
   }
   return rc;
 }
@@ -575,20 +613,23 @@ void pm_destroy(pm_ctx* c) {
     release(c->vbuf[v]);
     release(c->dt[v]);
     release(c->smooth[v]);
-    release(c->rof_p[v]);
-    release(c->rof_q[v]);
+    release(c->rofb[v].nd);
+    release(c->rofb[v].u);
+    for (int k = 0; k < 2; ++k) {
+      release(c->rofb[v].p[k]);
+      release(c->rofb[v].q[k]);
+    }
   }
   release(c->checked);
   release(c->scratch4);
   release(c->scratch4b);
   release(c->scratch1);
   release(c->counters);
-  release(c->nd);
-  release(c->rof_u);
-  if (c->side) {
-    (void)hipStreamSynchronize(c->side);
-    (void)hipStreamDestroy(c->side);
-  }
+  for (hipStream_t st : {c->side, c->smoothing})
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
   for (hipEvent_t e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -660,7 +701,7 @@ int pm_setup_prior(pm_ctx* c, const float* disparity, const float* normals) {
   // the seeded left planes start the left smooth field
   // (PMHuberStereo.cpp:473-477)
   if (c->prm.smoothing == PM_SMOOTH_HUBER &&
-      (rc = smooth_view(c, 0, state_ptr(c, 0, 0), c->theta)))
+      (rc = smooth_view(c, 0, state_ptr(c, 0, 0), c->theta, c->stream)))
     return rc;
   PM_HIP(c, hipStreamSynchronize(c->stream));  // host inputs may be freed
   c->cur = 0;
@@ -938,7 +979,7 @@ int pm_rof(pm_ctx* c, int view, const float* f, const float* lambda,
   float4* d_out = static_cast<float4*>(c->scratch4b.ptr);
   if ((rc = to_device(c, d_f, f, bytes))) return rc;
   const float4 lam = make_float4(lambda[0], lambda[1], lambda[2], lambda[3]);
-  if ((rc = rof(c, view, d_f, lam, iters, d_out))) return rc;
+  if ((rc = rof(c, view, d_f, lam, iters, d_out, c->stream))) return rc;
   return to_host(c, out, d_out, bytes);
 }
 
@@ -958,7 +999,7 @@ int pm_rof_device(pm_ctx* c, int view, const float* d_f, const float* lambda,
     return fail(c, PM_ERR_ARG, "d_f and d_out overlap");
   const float4 lam = make_float4(lambda[0], lambda[1], lambda[2], lambda[3]);
   return rof(c, view, reinterpret_cast<const float4*>(d_f), lam, iters,
-             reinterpret_cast<float4*>(d_out));
+             reinterpret_cast<float4*>(d_out), c->stream);
 }
 
 int pm_math_selftest(pm_ctx* c, const float* in, int n, float* out) {
