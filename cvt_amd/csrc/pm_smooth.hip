@@ -95,7 +95,12 @@ __device__ __forceinline__ Dual dual_of(const float4& u0, const float4& ux,
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const float pr = k < 3 ? pxyz : pw;
-    const float lo = r.lo[k] / pr, hi = r.hi[k] / pr;
+    // x / 1 is x exactly: the division only runs for a projected dual
+    float lo = r.lo[k], hi = r.hi[k];
+    if (pr != 1.0f) {
+      lo = lo / pr;
+      hi = hi / pr;
+    }
     r.lo[k] = D.x * lo + D.z * hi;
     r.hi[k] = D.y * lo + D.w * hi;
   }
