@@ -373,3 +373,23 @@ def test_flat_and_saturated_images_bitexact(oracle, kind):
             got = ctx.planes(view)
             assert same(got, want[1][view]), (view,
                                                first_diff(got, want[1][view]))
+
+
+@pytest.mark.parametrize("W,H,dmax", [(1280, 720, 128.0), (1920, 1080, 192.0)])
+def test_full_size_bitexact(oracle, synth, W, H, dmax):
+    """The 720p bench configuration and a 1080p pair at full size: the
+    initial fields, one iteration and the LR-checked disparity, bit for
+    bit (the oracle takes seconds per iteration on the host cores)."""
+    L, R, _ = synth.make_pair(W, H, cells=24, dmax=dmax, channels=4, seed=0)
+    r = 4
+    print("full size", W, H, flush=True)
+    ref = oracle.run(L, R, r, dmax, 1, maxdisp=1.0, maxdeg=180.0)
+    with cvt_amd.Context(radius=r, disparity_max=dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(1)
+        for view, key in ((0, "stateL"), (1, "stateR")):
+            got = ctx.planes(view)
+            assert same(got, ref[key]), (view, first_diff(got, ref[key]))
+        d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 180.0)
+    assert same(d, ref["disp"]), first_diff(d, ref["disp"])
