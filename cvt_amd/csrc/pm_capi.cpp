@@ -348,8 +348,9 @@ int split_rows(const pm_ctx* c) {
 }
 
 // Makes everything enqueued on the context's stream so far a predecessor
-// of the next half launched on either stream (all four events recorded
-// on the context's stream).
+// of the next half or smoothing launched on any stream (every event is
+// recorded on the context's stream).  The streams and events are made on
+// the first call.
 int split_rebase(pm_ctx* c) {
   if (!c->side) {
     PM_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
