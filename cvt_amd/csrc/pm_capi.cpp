@@ -229,10 +229,15 @@ int set_images(pm_ctx* c, const float* l, const float* r, int W, int H,
   if ((rc = use_device(c))) return rc;
   if ((rc = alloc_pair(c, W, H, channels))) return rc;
   const size_t row = (size_t)W * channels * sizeof(float);
-  PM_HIP(c, hipMemcpy2DAsync(c->img[0].ptr, row, l, (size_t)stride * 4, row,
-                             H, kind, c->stream));
-  PM_HIP(c, hipMemcpy2DAsync(c->img[1].ptr, row, r, (size_t)stride * 4, row,
-                             H, kind, c->stream));
+  const float* src[2] = {l, r};
+  for (int v = 0; v < 2; ++v) {
+    if ((size_t)stride * 4 == row)  // packed rows: one contiguous copy
+      PM_HIP(c, hipMemcpyAsync(c->img[v].ptr, src[v], row * H, kind,
+                               c->stream));
+    else
+      PM_HIP(c, hipMemcpy2DAsync(c->img[v].ptr, row, src[v],
+                                 (size_t)stride * 4, row, H, kind, c->stream));
+  }
   if (kind == hipMemcpyHostToDevice) PM_HIP(c, hipStreamSynchronize(c->stream));
   c->have_images = true;
   c->ready = false;
