@@ -393,3 +393,34 @@ def test_full_size_bitexact(oracle, synth, W, H, dmax):
             assert same(got, ref[key]), (view, first_diff(got, ref[key]))
         d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 180.0)
     assert same(d, ref["disp"]), first_diff(d, ref["disp"])
+
+
+def test_two_contexts_in_two_threads(synth):
+    """Contexts are independent: two threads running their own contexts at
+    once (each with its own streams and events) get what each gets alone."""
+    import threading
+    pairs = [synth.make_pair(211, 97, cells=6, dmax=20.0, channels=4,
+                             seed=s)[:2] for s in (21, 22)]
+
+    def run(L, R, out, k):
+        with cvt_amd.Context(radius=3, disparity_max=20.0) as ctx:
+            ctx.set_images(L, R)
+            ctx.setup()
+            ctx.iterate(3)
+            out[k] = (ctx.planes(0), ctx.planes(1),
+                      ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0))
+
+    alone = {}
+    for k, (L, R) in enumerate(pairs):
+        run(L, R, alone, k)
+    together = {}
+    threads = [threading.Thread(target=run, args=(L, R, together, k))
+               for k, (L, R) in enumerate(pairs)]
+    print("two threads", flush=True)
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for k in range(2):
+        for a, b in zip(alone[k], together[k]):
+            assert same(a, b), (k, first_diff(a, b))
