@@ -424,3 +424,41 @@ def test_two_contexts_in_two_threads(synth):
     for k in range(2):
         for a, b in zip(alone[k], together[k]):
             assert same(a, b), (k, first_diff(a, b))
+
+
+def test_strided_inputs_match_packed(synth):
+    """Rows with padding (stride > width x channels), from host and from
+    device memory, give the fields packed rows give."""
+    import torch
+    from cvt_amd.stereo import _ptr
+    L, R, _ = synth.make_pair(150, 40, cells=4, dmax=16.0, channels=4,
+                              seed=31)
+    H, W, C, pad = L.shape[0], L.shape[1], 4, 7
+
+    def padded(img):
+        buf = np.full((H, W * C + pad), np.nan, np.float32)
+        buf[:, :W * C] = img.reshape(H, W * C)
+        return buf
+
+    def fields(ctx):
+        ctx.setup()
+        ctx.iterate(2)
+        return ctx.planes(0), ctx.planes(1)
+
+    pl, pr = padded(L), padded(R)
+    print("strided inputs", flush=True)
+    with cvt_amd.Context(radius=2, disparity_max=16.0) as ctx:
+        ctx.set_images(L, R)
+        want = fields(ctx)
+        ctx._check(ctx._lib.pm_set_images(ctx._h, _ptr(pl), _ptr(pr), W, H,
+                                          C, W * C + pad))
+        from_host = fields(ctx)
+        dl = torch.from_numpy(pl).to("cuda:0")
+        dr = torch.from_numpy(pr).to("cuda:0")
+        torch.cuda.synchronize()
+        ctx.set_images_device(dl.data_ptr(), dr.data_ptr(), W, H, C,
+                              stride=W * C + pad)
+        from_device = fields(ctx)
+    for got in (from_host, from_device):
+        for a, b in zip(got, want):
+            assert same(a, b), first_diff(a, b)
