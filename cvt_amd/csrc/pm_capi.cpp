@@ -230,13 +230,32 @@ int set_images(pm_ctx* c, const float* l, const float* r, int W, int H,
   if ((rc = alloc_pair(c, W, H, channels))) return rc;
   const size_t row = (size_t)W * channels * sizeof(float);
   const float* src[2] = {l, r};
+  // Host rows go up as one contiguous copy per image: padded rows are
+  // packed on the host first (a row-by-row copy from pageable memory is
+  // several times slower).  Device rows use a 2D copy when padded.
+  std::vector<float> packed;
   for (int v = 0; v < 2; ++v) {
-    if ((size_t)stride * 4 == row)  // packed rows: one contiguous copy
-      PM_HIP(c, hipMemcpyAsync(c->img[v].ptr, src[v], row * H, kind,
-                               c->stream));
-    else
-      PM_HIP(c, hipMemcpy2DAsync(c->img[v].ptr, row, src[v],
-                                 (size_t)stride * 4, row, H, kind, c->stream));
+    const float* s = src[v];
+    if ((size_t)stride * 4 != row && kind == hipMemcpyHostToDevice) {
+      try {
+        packed.resize((size_t)W * channels * H);
+      } catch (const std::bad_alloc&) {
+        return fail(c, PM_ERR_NOMEM, "out of host memory");
+      }
+      for (int y = 0; y < H; ++y)
+        std::memcpy(packed.data() + (size_t)y * W * channels,
+                    s + (size_t)y * stride, row);
+      s = packed.data();
+      // the copy below reads `packed` before it is refilled for the
+      // next image
+      PM_HIP(c, hipMemcpyAsync(c->img[v].ptr, s, row * H, kind, c->stream));
+      PM_HIP(c, hipStreamSynchronize(c->stream));
+    } else if ((size_t)stride * 4 == row) {
+      PM_HIP(c, hipMemcpyAsync(c->img[v].ptr, s, row * H, kind, c->stream));
+    } else {
+      PM_HIP(c, hipMemcpy2DAsync(c->img[v].ptr, row, s, (size_t)stride * 4,
+                                 row, H, kind, c->stream));
+    }
   }
   if (kind == hipMemcpyHostToDevice) PM_HIP(c, hipStreamSynchronize(c->stream));
   c->have_images = true;
