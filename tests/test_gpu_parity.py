@@ -3,9 +3,9 @@
 The engine and the oracle evaluate the same operation sequence (see the
 arithmetic contract in cvt_amd/csrc/pm_device.h and oracle/pm_oracle.h),
 so integer and index results must match exactly and float results are
-compared bit for bit; the north-star bar (per-pixel cost within 1e-4
-relative, final disparity within 0.5 px on >= 99.5% of pixels) is asserted
-as well so a deviation is reported against it.
+compared bit for bit; the looser tolerance bar (per-pixel cost within
+1e-4 relative, final disparity within 0.5 px on >= 99.5% of pixels) is
+asserted as well so a deviation is reported against it.
 """
 import numpy as np
 import pytest
@@ -184,7 +184,7 @@ def test_iterations_bitexact(oracle, c1, rgba_small, which, radius, dmax,
             for view in (0, 1):
                 got = ctx.planes(view)
                 w = want[it][view]
-                # north-star bar first, then bit equality
+                # the tolerance bar first, then bit equality
                 assert cost_close(got[..., 3], w[..., 3]), (it, view)
                 assert same(got, w), (it, view, first_diff(got, w))
 
