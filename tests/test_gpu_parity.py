@@ -462,3 +462,27 @@ def test_strided_inputs_match_packed(synth):
     for got in (from_host, from_device):
         for a, b in zip(got, want):
             assert same(a, b), first_diff(a, b)
+
+
+def test_full_size_eight_iteration_invariants(synth):
+    """The 720p bench configuration over all 8 iterations, by properties
+    that hold at any size (plain PatchMatch, theta = 0): a pixel's stored
+    cost never rises from one iteration to the next, and it equals a fresh
+    evaluation of its stored plane (pm_cost), bit for bit."""
+    L, R, _ = synth.make_pair(1280, 720, cells=24, dmax=128.0, channels=4,
+                              seed=0)
+    print("full size invariants", flush=True)
+    with cvt_amd.Context(radius=4, disparity_max=128.0) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        prev = [ctx.planes(v)[..., 3] for v in (0, 1)]
+        for it in range(8):
+            ctx.iterate(1)
+            for v in (0, 1):
+                planes = ctx.planes(v)
+                w = planes[..., 3]
+                rose = np.argwhere(w > prev[v])
+                assert rose.size == 0, (it, v, len(rose), tuple(rose[0]))
+                fresh = ctx.cost(v, planes)
+                assert same(fresh, w), (it, v, first_diff(fresh, w))
+                prev[v] = w
