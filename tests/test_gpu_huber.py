@@ -157,3 +157,23 @@ def test_rof_device_matches_host(pair):
         with pytest.raises(cvt_amd.PMError):
             ctx.rof_device(0, df.data_ptr(), lam, 7, df.data_ptr())
     assert_same(got, want, "rof_device")
+
+
+def test_huber_full_size_bitexact(oracle, synth):
+    """The PM-Huber pipeline on the 720p bench pair: two coupled iterations
+    (propagates, both smoothings beside them, theta schedule), the smooth
+    fields and the disparity, bit for bit."""
+    L, R, _ = synth.make_pair(1280, 720, cells=24, dmax=128.0, channels=4,
+                              seed=0)
+    r, dmax, n = 4, 128.0, 2
+    print("huber full size", flush=True)
+    ref = oracle.run_huber(L, R, r, dmax, n, 1e-4, 1.0, 5.0)
+    with huber_ctx(r, dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(n)
+        got = {"stateL": ctx.planes(0), "stateR": ctx.planes(1),
+               "smoothL": ctx.smooth(0), "smoothR": ctx.smooth(1),
+               "disp": ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)}
+    for k, v in got.items():
+        assert_same(v, ref[k], k)
