@@ -4,14 +4,17 @@
 Workload (one step = one stereo pair, inputs resident in HBM): a seeded
 synthetic 1280x720 RGBA float32 pair with 24 slanted planar cells and
 disparities in 0-128 px; precompute (bilateral weights, gradients), random
-plane init of both views, 8 PatchMatch iterations (left then right view)
-with a 9x9 patch, LR check at 1 px (angle test off) and disparity
-conversion, all on the engine's stream.
+plane init of both views, 8 PatchMatch iterations (left then right view,
+one pm_iterate call) with a 9x9 patch, LR check at 1 px (angle test off)
+and disparity conversion, ordered on the engine's stream (the engine runs
+each view's propagate as two half launches on a second stream of its own
+and joins it back).
 
 Prints one JSON line with the throughput, the roofline of the dominant
 kernel (k_propagate: FP32 VALU bound, algorithmic FLOPs from an exact tap
-count) and the CPU baseline (the oracle on the host cores, timed on a
-bounded sample of the same workload).
+count, per view propagate) and the CPU baseline (the oracle on the host
+cores, timed on a bounded sample of the same workload).  --pipeline huber
+runs the PM-Huber pipeline instead and quotes the ROF kernel it adds.
 """
 import argparse
 import json
