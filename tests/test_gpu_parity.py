@@ -486,3 +486,34 @@ def test_full_size_eight_iteration_invariants(synth):
                 fresh = ctx.cost(v, planes)
                 assert same(fresh, w), (it, v, first_diff(fresh, w))
                 prev[v] = w
+
+
+def test_prior_init_full_size_bitexact(oracle, synth):
+    """The prior-seeded start (the hook a plane proposal feeds) on the 720p
+    bench pair: ground-truth disparity with holes seeds the left view, then
+    two iterations refine it; planes after each step bit for bit."""
+    L, R, gt = synth.make_pair(1280, 720, cells=24, dmax=128.0, channels=4,
+                               seed=0)
+    r, dmax = 4, 128.0
+    prior = gt.copy()
+    prior[::5, ::3] = 0.0                    # pixels without a prior
+    print("prior full size", flush=True)
+    tl, tr = oracle.prepare(L, r), oracle.prepare(R, r)
+    sl = oracle.init_prior(tl, tr, r, dmax, 1, prior,
+                           oracle.disparity_to_normal(prior))
+    sr = oracle.init(tr, tl, r, dmax, 0)
+    vb2 = oracle.viewbuf_empty(1280, 720)
+    want = []
+    for it in range(2):
+        sl, vb1 = oracle.propagate(tl, tr, r, dmax, 1, it, sl, vb2, sr)
+        sr, vb2 = oracle.propagate(tr, tl, r, dmax, 0, it, sr, vb1, sl)
+        want.append((sl, sr))
+    with cvt_amd.Context(radius=r, disparity_max=dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup_prior(prior)
+        for it in range(2):
+            ctx.iterate(1)
+            for v in (0, 1):
+                got = ctx.planes(v)
+                assert same(got, want[it][v]), (it, v,
+                                                 first_diff(got, want[it][v]))
