@@ -126,7 +126,11 @@ int pm_setup_prior(pm_ctx* ctx, const float* disparity, const float* normals);
  * smooths both views' (normal, depth) fields (normalDepth :668-675, then
  * denoiseDiffusionTensorROF_PDD) and advances theta.  Exactly n iterations
  * (the reference's `iterations |= 1` lives in the C++ wrapper's
- * disparityMap). */
+ * disparityMap).  Internally the call forks work to streams the context
+ * owns (each view's propagate runs as two halves; PM-Huber smoothing runs
+ * beside the other view's propagate) and joins them back into the
+ * context's stream before it returns, so the call stays ordered on that
+ * stream like any other. */
 int pm_iterate(pm_ctx* ctx, int n);
 int pm_iteration(const pm_ctx* ctx);
 
