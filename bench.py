@@ -70,6 +70,24 @@ def profiled_traffic():
     return float(prof["hbm_bytes_per_propagate"])
 
 
+def host_cpu():
+    """The host the CPU baseline ran on: CPU model, logical CPUs seen by
+    the process, and the oracle's thread count."""
+    from oracle import oracle as O
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+             else os.cpu_count())
+    return {"cpu_model": model, "nproc": nproc, "threads": O.num_threads()}
+
+
 def cpu_baseline(L, R, gt):
     """The oracle (C, OpenMP on all host cores) on a bounded sample: the
     full setup, SAMPLE_ITERS of the eight iterations and the
@@ -93,7 +111,7 @@ def cpu_baseline(L, R, gt):
     per_pair = (t1 - t0) + ITERS * per_iter + (t3 - t2)
     del d, np
     return {"value": 1.0 / per_pair, "unit": "pairs/s",
-            "cores": O.num_threads(), "kind": "port",
+            "cores": O.num_threads(), "kind": "port", "host": host_cpu(),
             "sample": "1280x720 pair: setup + %d of %d iterations + LR "
                       "check timed (%.1f s); per pair = setup %.2f s + %d x "
                       "iteration %.2f s + post %.3f s" %
@@ -164,7 +182,7 @@ def cpu_baseline_huber(L, R):
     t2 = time.perf_counter()
     per_pair = (t1 - t0) + ITERS * (t2 - t1)
     return {"value": 1.0 / per_pair, "unit": "pairs/s",
-            "cores": O.num_threads(), "kind": "port",
+            "cores": O.num_threads(), "kind": "port", "host": host_cpu(),
             "sample": "1280x720 pair, PM-Huber: setup + 1 of %d iterations "
                       "timed (%.1f s); per pair = setup %.2f s + %d x "
                       "iteration %.2f s" % (ITERS, t2 - t0, t1 - t0, ITERS,
