@@ -4,7 +4,8 @@
 // the PM-Huber smoothing coupling (cvt/vision/PMHuberStereo.cpp:349-404,
 // 406-581,583-644, cvt/vision/PDOptCL.cpp:134-155,1114-1168) over the HIP
 // kernels in pm_engine.hip.  Every device buffer is owned by the context
-// and sized for the current image pair; all work is queued on one stream.
+// and sized for the current image pair.  Work is queued on the context's
+// stream; pm_iterate forks onto streams of its own and joins them back.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +23,9 @@
 namespace {
 
 constexpr int kMaxRadius = 16;
+
+// bands of block rows a view's propagate is split into (split schedule)
+constexpr int kSplitParts = 2;
 
 struct DevBuf {
   void* ptr = nullptr;
@@ -67,14 +71,17 @@ struct pm_ctx {
   float theta = 0.0f;   // theta of the next iteration
   int schedule_n = 1;   // N of the theta schedule of the running pm_iterate
 
-  // Split schedule of the iterations: each view's propagate runs as a top
-  // and a bottom half, the top halves on `stream`, the bottom halves on
-  // `side`, so that one half's last work-groups overlap the next half's
-  // first; with PM_SMOOTH_HUBER each view's smoothing runs on `smoothing`
-  // beside the other view's propagate.  ev[k] marks the end of the latest
-  // launch of half k (or of view k - 4's smoothing).
-  hipStream_t side = nullptr, smoothing = nullptr;
-  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // Split schedule of the iterations: each view's propagate runs as
+  // kSplitParts bands of block rows, band 0 on `stream` and band k > 0 on
+  // side[k - 1], so that one band's last work-groups overlap other bands'
+  // work; with PM_SMOOTH_HUBER each view's smoothing runs on `smoothing`
+  // beside the other view's propagate.  ev_band[v][k] marks the end of
+  // the latest launch of view v's band k, ev_smooth[v] that of view v's
+  // latest smoothing.
+  hipStream_t side[kSplitParts - 1] = {};
+  hipStream_t smoothing = nullptr;
+  hipEvent_t ev_band[2][kSplitParts] = {};
+  hipEvent_t ev_smooth[2] = {};
 };
 
 namespace {
@@ -362,66 +369,93 @@ int smooth_view(pm_ctx* c, int view, const float4* planes, float theta,
              f4buf(c->smooth[view]), st);
 }
 
-// Halves and smoothings of the split schedule (ctx.ev indices)
-enum { kTopL = 0, kBotL = 1, kTopR = 2, kBotR = 3, kSmL = 4, kSmR = 5 };
+static_assert(kSplitParts >= 2, "the split schedule needs two bands");
 
-// block rows of the top half; 0 when the image is too short to split
-int split_rows(const pm_ctx* c) {
-  const int nb = (c->geo.H + pm::kBlockRows - 1) / pm::kBlockRows;
-  return nb < 2 ? 0 : nb / 2;
+// Rows above and below its own that a band reads of its view's previous
+// field: the random neighbours' row offsets lie in [-4, 5]
+// (pmhstereo.cl:981-991, NUMRNDSAMPLERADIUS 5).
+constexpr int kBandReach = 5;
+
+int block_rows(const pm_ctx* c) {
+  return (c->geo.H + pm::kBlockRows - 1) / pm::kBlockRows;
+}
+
+// true when every band of the split schedule gets a block row
+bool can_split(const pm_ctx* c) { return block_rows(c) >= kSplitParts; }
+
+// first block row of band k (band k ends where band k + 1 starts)
+int band_first(const pm_ctx* c, int k) {
+  return (int)((long long)block_rows(c) * k / kSplitParts);
+}
+
+// image rows [*r0, *r1) of band k
+void band_rows(const pm_ctx* c, int k, int* r0, int* r1) {
+  *r0 = band_first(c, k) * pm::kBlockRows;
+  *r1 = std::min(c->geo.H, band_first(c, k + 1) * pm::kBlockRows);
+}
+
+hipStream_t band_stream(const pm_ctx* c, int k) {
+  return k == 0 ? c->stream : c->side[k - 1];
 }
 
 // Makes everything enqueued on the context's stream so far a predecessor
-// of the next half or smoothing launched on any stream (every event is
-// recorded on the context's stream).  The streams and events are made on
-// the first call.
+// of the next band or smoothing launched on any stream (every event is
+// recorded on the context's stream).  Streams and events are made on
+// first use.
 int split_rebase(pm_ctx* c) {
-  if (!c->side) {
-    PM_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  for (hipStream_t& s : c->side)
+    if (!s) PM_HIP(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  if (!c->smoothing)
     PM_HIP(c, hipStreamCreateWithFlags(&c->smoothing, hipStreamNonBlocking));
-    for (hipEvent_t& e : c->ev)
-      PM_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (int v = 0; v < 2; ++v) {
+    for (hipEvent_t& e : c->ev_band[v]) {
+      if (!e) PM_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      PM_HIP(c, hipEventRecord(e, c->stream));
+    }
+    hipEvent_t& e = c->ev_smooth[v];
+    if (!e) PM_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    PM_HIP(c, hipEventRecord(e, c->stream));
   }
-  for (hipEvent_t e : c->ev) PM_HIP(c, hipEventRecord(e, c->stream));
   return PM_OK;
 }
 
-// One view's propagate as two halves.  Each half reads, besides its own
-// rows, the view's previous field within 5 rows of them (the random
-// neighbours) and writes its rows of the new field and of the scatter
-// slots; the other view's field and slots it reads are on its own rows.
-// So a half waits for the other half of the same view's previous
-// propagate (rows it reads, and rows of the buffer it overwrites that
-// that half still reads) and follows its own stream otherwise: the top
-// half of the right view follows the top half of the left view, and so
-// on.  Both waits are enqueued before this propagate's events are
-// recorded, so they refer to the previous propagate of the view.
+// One view's propagate as kSplitParts bands of block rows.  A band reads,
+// besides its own rows, the view's previous field within kBandReach rows
+// of them (the random neighbours) and writes its rows of the new field and
+// of the scatter slots; the other view's field and slots it reads are on
+// its own rows.  So band k waits for every other band j of the same
+// view's previous propagate whose rows lie within kBandReach of its own
+// (rows it reads, and rows of the buffer it overwrites that band j still
+// reads: the same condition) and follows its own stream otherwise: band k
+// of the right view follows band k of the left view, and so on.  All
+// waits are enqueued before this propagate records its events, so they
+// refer to the view's previous propagate.
 int propagate_split(pm_ctx* c, const pm::PropagateArgs& a, int view,
                     unsigned long long* vb, unsigned long long* counters) {
-  const int nbt = split_rows(c);
-  const int rows_top = nbt * pm::kBlockRows;
-  pm::Geometry top = c->geo, bot = c->geo;
-  top.by0 = 0;
-  top.nby = nbt;
-  bot.by0 = nbt;
-  bot.nby = (c->geo.H + pm::kBlockRows - 1) / pm::kBlockRows - nbt;
-  hipEvent_t& ev_top = c->ev[view == 0 ? kTopL : kTopR];
-  hipEvent_t& ev_bot = c->ev[view == 0 ? kBotL : kBotR];
-  PM_HIP(c, hipStreamWaitEvent(c->side, ev_top, 0));
-  PM_HIP(c, hipStreamWaitEvent(c->stream, ev_bot, 0));
-  if (c->prm.smoothing == PM_SMOOTH_HUBER) {
-    // the view's smooth field, from the smoothing after its last propagate
-    hipEvent_t sm = c->ev[view == 0 ? kSmL : kSmR];
-    PM_HIP(c, hipStreamWaitEvent(c->side, sm, 0));
-    PM_HIP(c, hipStreamWaitEvent(c->stream, sm, 0));
+  hipEvent_t* ev = c->ev_band[view];
+  for (int k = 0; k < kSplitParts; ++k) {
+    int r0, r1;
+    band_rows(c, k, &r0, &r1);
+    for (int j = 0; j < kSplitParts; ++j) {
+      int s0, s1;
+      band_rows(c, j, &s0, &s1);
+      if (j != k && s0 < r1 + kBandReach && s1 > r0 - kBandReach)
+        PM_HIP(c, hipStreamWaitEvent(band_stream(c, k), ev[j], 0));
+    }
+    if (c->prm.smoothing == PM_SMOOTH_HUBER)  // the view's smooth field
+      PM_HIP(c, hipStreamWaitEvent(band_stream(c, k), c->ev_smooth[view], 0));
   }
-  PM_HIP(c, pm::launch_viewbuf_clear_rows(c->stream, c->geo, vb, 0, rows_top));
-  PM_HIP(c, pm::launch_propagate(c->stream, top, a, counters));
-  PM_HIP(c, hipEventRecord(ev_top, c->stream));
-  PM_HIP(c, pm::launch_viewbuf_clear_rows(c->side, c->geo, vb, rows_top,
-                                          c->geo.H - rows_top));
-  PM_HIP(c, pm::launch_propagate(c->side, bot, a, counters));
-  PM_HIP(c, hipEventRecord(ev_bot, c->side));
+  for (int k = 0; k < kSplitParts; ++k) {
+    int r0, r1;
+    band_rows(c, k, &r0, &r1);
+    pm::Geometry g = c->geo;
+    g.by0 = band_first(c, k);
+    g.nby = band_first(c, k + 1) - g.by0;
+    const hipStream_t st = band_stream(c, k);
+    PM_HIP(c, pm::launch_viewbuf_clear_rows(st, c->geo, vb, r0, r1 - r0));
+    PM_HIP(c, pm::launch_propagate(st, g, a, counters));
+    PM_HIP(c, hipEventRecord(ev[k], st));
+  }
   return PM_OK;
 }
 
@@ -437,21 +471,19 @@ int propagate_view(pm_ctx* c, const pm::PropagateArgs& a, int view,
 }
 
 // The smoothing of a view's new field (PMHuberStereo.cpp:552-567).  In the
-// split schedule it runs on the smoothing stream once both halves of the
-// view's propagate are done, beside the other view's propagate; the next
+// split schedule it runs on the smoothing stream once every band of the
+// view's propagate is done, beside the other view's propagate; the next
 // propagate of the same view, which reads the smooth field, waits for it
-// (propagate_view), and so does every later user of the field buffers it
+// (propagate_split), and so does every later user of the field buffers it
 // reads, since they follow that propagate.
 int smooth_after(pm_ctx* c, int view, const float4* planes, float theta,
                  bool split) {
   if (!split) return smooth_view(c, view, planes, theta, c->stream);
-  PM_HIP(c, hipStreamWaitEvent(c->smoothing,
-                               c->ev[view == 0 ? kTopL : kTopR], 0));
-  PM_HIP(c, hipStreamWaitEvent(c->smoothing,
-                               c->ev[view == 0 ? kBotL : kBotR], 0));
+  for (hipEvent_t e : c->ev_band[view])
+    PM_HIP(c, hipStreamWaitEvent(c->smoothing, e, 0));
   int rc;
   if ((rc = smooth_view(c, view, planes, theta, c->smoothing))) return rc;
-  PM_HIP(c, hipEventRecord(c->ev[view == 0 ? kSmL : kSmR], c->smoothing));
+  PM_HIP(c, hipEventRecord(c->ev_smooth[view], c->smoothing));
   return PM_OK;
 }
 
@@ -506,20 +538,24 @@ int one_iteration(pm_ctx* c, unsigned long long* counters, bool split) {
 }
 
 // n iterations; with the split schedule the context's stream finally
-// waits for the last bottom half, so later work on it sees every result
+// waits for the other streams, so later work on it sees every result
 int iterate_n(pm_ctx* c, int n, unsigned long long* counters) {
-  const bool split = n > 0 && split_rows(c) > 0;
+  const bool split = n > 0 && can_split(c);
   int rc;
   if (split && (rc = split_rebase(c))) return rc;
   for (int i = 0; i < n; ++i)
     if ((rc = one_iteration(c, counters, split))) break;
   if (split) {
     // joined even after a failed launch, so that no work is left behind
-    // unordered on the other streams: the last bottom half and the last
-    // smoothing (the smoothing stream runs its launches in order)
-    for (int k : {kBotR, kSmR}) {
-      const hipError_t e = hipStreamWaitEvent(c->stream, c->ev[k], 0);
-      if (!rc && e != hipSuccess) rc = hip_fail(c, e, "hipStreamWaitEvent");
+    // unordered on the other streams: each records how far it has got and
+    // the context's stream waits for that (the events are recorded afresh
+    // by the next split_rebase)
+    for (int k = 0; k < kSplitParts; ++k) {
+      const hipStream_t st = k == 0 ? c->smoothing : c->side[k - 1];
+      hipEvent_t ev = k == 0 ? c->ev_smooth[1] : c->ev_band[1][k];
+      hipError_t e = hipEventRecord(ev, st);
+      if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, ev, 0);
+      if (!rc && e != hipSuccess) rc = hip_fail(c, e, "join of the streams");
     }
   }
   return rc;
@@ -650,13 +686,18 @@ void pm_destroy(pm_ctx* c) {
   release(c->scratch4b);
   release(c->scratch1);
   release(c->counters);
-  for (hipStream_t st : {c->side, c->smoothing})
+  std::vector<hipStream_t> streams(c->side, c->side + kSplitParts - 1);
+  streams.push_back(c->smoothing);
+  for (hipStream_t st : streams)
     if (st) {
       (void)hipStreamSynchronize(st);
       (void)hipStreamDestroy(st);
     }
-  for (hipEvent_t e : c->ev)
-    if (e) (void)hipEventDestroy(e);
+  for (int v = 0; v < 2; ++v) {
+    for (hipEvent_t e : c->ev_band[v])
+      if (e) (void)hipEventDestroy(e);
+    if (c->ev_smooth[v]) (void)hipEventDestroy(c->ev_smooth[v]);
+  }
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

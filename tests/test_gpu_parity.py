@@ -488,6 +488,35 @@ def test_full_size_eight_iteration_invariants(synth):
                 prev[v] = w
 
 
+@pytest.mark.parametrize("pp", [cvt_amd.DISPARITY_PP_LRCHECK,
+                                cvt_amd.DISPARITY_PP_LRCHECK_FILL])
+def test_full_size_postprocessing_bitexact(oracle, synth, pp):
+    """LR check, hole filling and normals on the 720p bench pair's fields
+    after all 8 iterations, the fields the bench turns into its output:
+    each is a pure function of the two plane fields, so the oracle applied
+    to the GPU's own fields must give the GPU's outputs bit for bit."""
+    L, R, _ = synth.make_pair(1280, 720, cells=24, dmax=128.0, channels=4,
+                              seed=0)
+    print("full size post-processing", pp, flush=True)
+    with cvt_amd.Context(radius=4, disparity_max=128.0) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(8)
+        sl, sr = ctx.planes(0), ctx.planes(1)
+        d, chk = ctx.disparity(pp, 1.0, 5.0, checked=True)
+        nrm = ctx.normals(pp)
+    want_chk = oracle.consistency(sl, sr, 1.0, 5.0)
+    rejected = (want_chk[..., 3] == -1.0).mean()     # the check's holes
+    assert 0.001 < rejected < 0.95, rejected
+    assert same(chk, want_chk), first_diff(chk, want_chk)
+    want_d = (oracle.to_disparity(want_chk)
+              if pp == cvt_amd.DISPARITY_PP_LRCHECK
+              else oracle.fill_depthmap(want_chk))
+    assert same(d, want_d), first_diff(d, want_d)
+    want_n = oracle.fill_normalmap(want_chk)
+    assert same(nrm, want_n), first_diff(nrm, want_n)
+
+
 def test_prior_init_full_size_bitexact(oracle, synth):
     """The prior-seeded start (the hook a plane proposal feeds) on the 720p
     bench pair: ground-truth disparity with holes seeds the left view, then
