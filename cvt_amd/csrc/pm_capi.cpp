@@ -542,6 +542,16 @@ int one_iteration(pm_ctx* c, unsigned long long* counters, bool split) {
 int iterate_n(pm_ctx* c, int n, unsigned long long* counters) {
   const bool split = n > 0 && can_split(c);
   int rc;
+  if (c->prm.smoothing == PM_SMOOTH_HUBER) {
+    // Capturing the PM-Huber iteration into a graph ended with the process
+    // crashing inside the runtime's end of capture (plain PatchMatch
+    // captures and replays correctly); refused before anything is queued.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    PM_HIP(c, hipStreamIsCapturing(c->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(c, PM_ERR_ARG, "pm_iterate with PM_SMOOTH_HUBER cannot "
+                  "be recorded by stream capture");
+  }
   if (split && (rc = split_rebase(c))) return rc;
   for (int i = 0; i < n; ++i)
     if ((rc = one_iteration(c, counters, split))) break;
@@ -549,8 +559,11 @@ int iterate_n(pm_ctx* c, int n, unsigned long long* counters) {
     // joined even after a failed launch, so that no work is left behind
     // unordered on the other streams: each records how far it has got and
     // the context's stream waits for that (the events are recorded afresh
-    // by the next split_rebase)
-    for (int k = 0; k < kSplitParts; ++k) {
+    // by the next split_rebase).  The smoothing stream joins only when it
+    // ran something: otherwise it never waited for the context's stream,
+    // and under stream capture its record would lie outside the capture.
+    const bool huber = c->prm.smoothing == PM_SMOOTH_HUBER;
+    for (int k = huber ? 0 : 1; k < kSplitParts; ++k) {
       const hipStream_t st = k == 0 ? c->smoothing : c->side[k - 1];
       hipEvent_t ev = k == 0 ? c->ev_smooth[1] : c->ev_band[1][k];
       hipError_t e = hipEventRecord(ev, st);
