@@ -375,11 +375,14 @@ def test_flat_and_saturated_images_bitexact(oracle, kind):
                                                first_diff(got, want[1][view]))
 
 
-@pytest.mark.parametrize("W,H,dmax", [(1280, 720, 128.0), (1920, 1080, 192.0)])
+@pytest.mark.parametrize("W,H,dmax", [(1280, 720, 128.0), (1920, 1080, 192.0),
+                                      (1242, 375, 128.0)])
 def test_full_size_bitexact(oracle, synth, W, H, dmax):
-    """The 720p bench configuration and a 1080p pair at full size: the
-    initial fields, one iteration and the LR-checked disparity, bit for
-    bit (the oracle takes seconds per iteration on the host cores)."""
+    """The 720p bench configuration, a 1080p pair and a KITTI-sized pair
+    (1242 x 375: partial work-groups at the right edge and a partial last
+    block row) at full size: the initial fields, one iteration and the
+    LR-checked disparity, bit for bit (the oracle takes seconds per
+    iteration on the host cores)."""
     L, R, _ = synth.make_pair(W, H, cells=24, dmax=dmax, channels=4, seed=0)
     r = 4
     print("full size", W, H, flush=True)
