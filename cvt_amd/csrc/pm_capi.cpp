@@ -82,6 +82,8 @@ struct pm_ctx {
   hipStream_t smoothing = nullptr;
   hipEvent_t ev_band[2][kSplitParts] = {};
   hipEvent_t ev_smooth[2] = {};
+  // LDS the propagate's work-groups ask for at least (pm_create)
+  size_t prop_lds_floor = 0;
 };
 
 namespace {
@@ -453,7 +455,7 @@ int propagate_split(pm_ctx* c, const pm::PropagateArgs& a, int view,
     g.nby = band_first(c, k + 1) - g.by0;
     const hipStream_t st = band_stream(c, k);
     PM_HIP(c, pm::launch_viewbuf_clear_rows(st, c->geo, vb, r0, r1 - r0));
-    PM_HIP(c, pm::launch_propagate(st, g, a, counters));
+    PM_HIP(c, pm::launch_propagate(st, g, a, counters, c->prop_lds_floor));
     PM_HIP(c, hipEventRecord(ev[k], st));
   }
   return PM_OK;
@@ -466,7 +468,8 @@ int propagate_view(pm_ctx* c, const pm::PropagateArgs& a, int view,
                    bool split) {
   if (split) return propagate_split(c, a, view, vb, counters);
   PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vb));
-  PM_HIP(c, pm::launch_propagate(c->stream, c->geo, a, counters));
+  PM_HIP(c, pm::launch_propagate(c->stream, c->geo, a, counters,
+                                 c->prop_lds_floor));
   return PM_OK;
 }
 
@@ -623,6 +626,30 @@ int postprocess(pm_ctx* c, int pp, float maxdispdiff, float maxdeg,
   return PM_OK;
 }
 
+// With PM_SMOOTH_HUBER each view's smoothing runs beside the other view's
+// propagate.  Four propagate work-groups per CU (the most their VGPRs
+// allow) leave no room for a k_rof_steps work-group, which then gets a CU
+// only when propagate work-groups retire.  Asking for just over a quarter
+// of a CU's LDS per propagate work-group caps them at three per CU and
+// leaves LDS and VGPRs (3 x 112 + 120 of 512 per lane) for one smoothing
+// work-group beside them.  Measured on the 720p pair: 8 coupled
+// iterations 64.4 -> 61.5 ms.  Plain PatchMatch keeps four per CU (three
+// would cost it 4 %).  The floor is kept only when three such work-groups
+// and one k_rof_steps work-group fit the CU's LDS, allocated in 512-byte
+// granules.
+int share_cus_with_smoothing(pm_ctx* c) {
+  int lds_cu = 0;
+  PM_HIP(c, hipDeviceGetAttribute(
+                &lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                c->prm.device));
+  const size_t floor = (size_t)lds_cu / 4 + 64;
+  const size_t granule = 512;
+  const size_t held = (floor + granule - 1) / granule * granule;
+  if (3 * held + pm::rof_steps_lds_bytes() <= (size_t)lds_cu)
+    c->prop_lds_floor = floor;
+  return PM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -663,6 +690,8 @@ int pm_create(const pm_params* p, pm_ctx** out) {
   }
   *out = c;
   if ((rc = use_device(c))) return rc;
+  if (p->smoothing == PM_SMOOTH_HUBER && (rc = share_cus_with_smoothing(c)))
+    return rc;
   if (p->stream) {
     c->stream = static_cast<hipStream_t>(p->stream);
   } else {

@@ -80,10 +80,13 @@ struct PropagateArgs {
 };
 
 // counters (optional, 2 x u64): cost taps and KLT taps evaluated, counted
-// with the reference's per-tap early exit
+// with the reference's per-tap early exit.  lds_floor: the work-groups ask
+// for at least this much LDS (only the tile's share is used), which caps
+// how many share a CU.
 hipError_t launch_propagate(hipStream_t st, const Geometry& g,
                             const PropagateArgs& a,
-                            unsigned long long* counters = nullptr);
+                            unsigned long long* counters = nullptr,
+                            size_t lds_floor = 0);
 
 hipError_t launch_scatter(hipStream_t st, const Geometry& g,
                           const float4* states, int lr,
@@ -126,6 +129,8 @@ hipError_t launch_normal_depth(hipStream_t st, const Geometry& g,
 // nsteps <= kRofStepsMax diffusion-tensor ROF steps in one launch (u, q,
 // f, the tensor: one float4 per pixel; p: two)
 constexpr int kRofStepsMax = 2;
+// LDS of one k_rof_steps work-group (bytes)
+size_t rof_steps_lds_bytes();
 hipError_t launch_rof_steps(hipStream_t st, const Geometry& g, int nsteps,
                             const float4* u, const float4* p, const float4* q,
                             const float4* f, const float4* dt, float4 lambda,

@@ -18,6 +18,7 @@
 //
 // Per-pixel work follows cvt/cl/kernel/pmhstereo.cl (cited per kernel);
 // the arithmetic contract is in pm_device.h.
+#include <algorithm>
 #include <climits>
 
 #include "pm_device.h"
@@ -964,9 +965,18 @@ hipError_t launch_viewbuf_clear(hipStream_t st, const Geometry& g,
 
 hipError_t launch_propagate(hipStream_t st, const Geometry& g,
                             const PropagateArgs& a,
-                            unsigned long long* counters) {
-  if (counters) PM_LAUNCH_TILE(k_propagate<true>, g, st, g, a, counters);
-  PM_LAUNCH_TILE(k_propagate<false>, g, st, g, a, counters);
+                            unsigned long long* counters, size_t lds_floor) {
+  const size_t lds = std::max(block_lds_bytes(g.R), lds_floor);
+  if (counters) {
+    const hipError_t e = allow_lds(k_propagate<true>, lds);
+    if (e != hipSuccess) return e;
+    PM_LAUNCH(k_propagate<true>, grid_for(g), kBlockTile, lds, st, g, a,
+              counters);
+  }
+  const hipError_t e = allow_lds(k_propagate<false>, lds);
+  if (e != hipSuccess) return e;
+  PM_LAUNCH(k_propagate<false>, grid_for(g), kBlockTile, lds, st, g, a,
+            counters);
 }
 
 hipError_t launch_scatter(hipStream_t st, const Geometry& g,

@@ -277,6 +277,10 @@ hipError_t launch_normal_depth(hipStream_t st, const Geometry& g,
             st, g.W, g.H, states, dispscale, lr, out);
 }
 
+size_t rof_steps_lds_bytes() {
+  return (size_t)3 * kRofRH * kRofRW * sizeof(float4);  // su and sp
+}
+
 hipError_t launch_rof_steps(hipStream_t st, const Geometry& g, int nsteps,
                             const float4* u, const float4* p, const float4* q,
                             const float4* f, const float4* dt, float4 lambda,
