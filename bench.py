@@ -46,28 +46,35 @@ PEAK_HBM_GBS = 8000.0
 HBM_PROFILE = os.path.join(ROOT, "profiles",
                            "k_propagate_1280x720_rgba_r4_hbm.json")
 KERNEL_SOURCES = ("cvt_amd/csrc/pm_engine.hip", "cvt_amd/csrc/pm_device.h")
+# the same for one k_rof_steps launch of the PM-Huber pipeline
+ROF_HBM_PROFILE = os.path.join(ROOT, "profiles",
+                               "k_rof_steps_1280x720_rgba_huber_hbm.json")
+ROF_KERNEL_SOURCES = ("cvt_amd/csrc/pm_smooth.hip", "cvt_amd/csrc/pm_device.h",
+                      "cvt_amd/csrc/pm_engine.h")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic():
-    """Calibrated HBM bytes per view propagate, or None when the kernel
-    sources changed since the counters were collected."""
+def profiled_traffic(path=HBM_PROFILE, sources=KERNEL_SOURCES,
+                     key="hbm_bytes_per_propagate"):
+    """Calibrated HBM bytes (per view propagate by default) from a
+    profile file, or None when the kernel sources changed since the
+    counters were collected."""
     import hashlib
     try:
-        with open(HBM_PROFILE) as f:
+        with open(path) as f:
             prof = json.load(f)
         h = hashlib.sha256()
-        for rel in KERNEL_SOURCES:
+        for rel in sources:
             with open(os.path.join(ROOT, rel), "rb") as f:
                 h.update(f.read())
     except OSError:
         return None
     if h.hexdigest() != prof.get("source_sha256"):
         return None
-    return float(prof["hbm_bytes_per_propagate"])
+    return float(prof[key])
 
 
 def host_cpu():
@@ -366,7 +373,10 @@ def main():
         result["roofline"] = {
             "bound": "hbm", "kernel": "k_rof_steps", "achieved": gbs,
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
-            "traffic": None, "avg_launch_ms": rof_ms,
+            "traffic": profiled_traffic(ROF_HBM_PROFILE, ROF_KERNEL_SOURCES,
+                                        "hbm_bytes_per_launch"),
+            "traffic_unit": "bytes per k_rof_steps launch",
+            "avg_launch_ms": rof_ms,
             "steps_per_launch": ROF_STEPS_PER_LAUNCH,
             "iteration_ms": float(np.mean(iter_ms))}
     if not args.no_cpu_baseline:
