@@ -627,16 +627,16 @@ int postprocess(pm_ctx* c, int pp, float maxdispdiff, float maxdeg,
 }
 
 // With PM_SMOOTH_HUBER each view's smoothing runs beside the other view's
-// propagate.  Four propagate work-groups per CU (the most their VGPRs
-// allow) leave no room for a k_rof_steps work-group, which then gets a CU
-// only when propagate work-groups retire.  Asking for just over a quarter
-// of a CU's LDS per propagate work-group caps them at three per CU and
-// leaves LDS and VGPRs (3 x 112 + 120 of 512 per lane) for one smoothing
-// work-group beside them.  Measured on the 720p pair: 8 coupled
-// iterations 64.4 -> 61.5 ms.  Plain PatchMatch keeps four per CU (three
-// would cost it 4 %).  The floor is kept only when three such work-groups
-// and one k_rof_steps work-group fit the CU's LDS, allocated in 512-byte
-// granules.
+// propagate.  Propagate work-groups fill a CU's VGPRs (five per CU at 96
+// VGPRs each) and leave no room for a k_rof_steps work-group, which then
+// gets a CU only when propagate work-groups retire.  Asking for just over
+// a quarter of a CU's LDS per propagate work-group caps them at three per
+// CU and leaves LDS and VGPRs (3 x 96 + 120 of 512 per lane) for one
+// smoothing work-group beside them.  Measured on the 720p pair: 8 coupled
+// iterations 64.4 -> 61.5 ms (propagate at four per CU then).  Plain
+// PatchMatch keeps the CU to itself (three per CU would cost it 4 %).
+// The floor is kept only when three such work-groups and one k_rof_steps
+// work-group fit the CU's LDS, allocated in 512-byte granules.
 int share_cus_with_smoothing(pm_ctx* c) {
   int lds_cu = 0;
   PM_HIP(c, hipDeviceGetAttribute(
