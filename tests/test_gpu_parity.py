@@ -189,6 +189,40 @@ def test_iterations_bitexact(oracle, c1, rgba_small, which, radius, dmax,
                 assert same(got, w), (it, view, first_diff(got, w))
 
 
+@pytest.mark.parametrize("cw", [0.0, 5.0, 30.0])
+def test_color_weight_bitexact(oracle, rgba_small, cw):
+    """The support weights' colour factor (CENSUS_COLOR_WEIGHT, 12 in the
+    reference) as a parameter: 0 makes every weight exactly 1, 30 drives
+    most towards underflow.  Texels (their weight sums), the initial fields
+    and two iterations, bit for bit."""
+    L, R, _ = rgba_small
+    r, dmax = 4, 24.0
+    tl = oracle.prepare(L, r, color_weight=cw)
+    tr = oracle.prepare(R, r, color_weight=cw)
+    sl = oracle.init(tl, tr, r, dmax, 1, color_weight=cw)
+    sr = oracle.init(tr, tl, r, dmax, 0, color_weight=cw)
+    with cvt_amd.Context(radius=r, disparity_max=dmax,
+                         color_weight=cw) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for view, want in ((0, tl), (1, tr)):
+            got = ctx.texels(view)
+            assert same(got, want), (view, first_diff(got, want))
+        for view, want in ((0, sl), (1, sr)):
+            got = ctx.planes(view)
+            assert same(got, want), (view, first_diff(got, want))
+        vb2 = oracle.viewbuf_empty(L.shape[1], L.shape[0])
+        for it in range(2):
+            sl, vb1 = oracle.propagate(tl, tr, r, dmax, 1, it, sl, vb2, sr,
+                                       color_weight=cw)
+            sr, vb2 = oracle.propagate(tr, tl, r, dmax, 0, it, sr, vb1, sl,
+                                       color_weight=cw)
+            ctx.iterate(1)
+            for view, want in ((0, sl), (1, sr)):
+                got = ctx.planes(view)
+                assert same(got, want), (it, view, first_diff(got, want))
+
+
 def test_pipeline_matches_oracle_run(oracle, c1):
     """C1 configuration end to end: 320x240 gray, 11x11 patch, 3
     iterations, LR check at 1 px (angle test off)."""
@@ -375,17 +409,18 @@ def test_flat_and_saturated_images_bitexact(oracle, kind):
                                                first_diff(got, want[1][view]))
 
 
-@pytest.mark.parametrize("W,H,dmax", [(1280, 720, 128.0), (1920, 1080, 192.0),
-                                      (1242, 375, 128.0)])
-def test_full_size_bitexact(oracle, synth, W, H, dmax):
-    """The 720p bench configuration, a 1080p pair and a KITTI-sized pair
+@pytest.mark.parametrize("W,H,dmax,C,r", [(1280, 720, 128.0, 4, 4),
+                                          (1920, 1080, 192.0, 4, 4),
+                                          (1242, 375, 128.0, 4, 4),
+                                          (1280, 720, 128.0, 1, 5)])
+def test_full_size_bitexact(oracle, synth, W, H, dmax, C, r):
+    """The 720p bench configuration, a 1080p pair, a KITTI-sized pair
     (1242 x 375: partial work-groups at the right edge and a partial last
-    block row) at full size: the initial fields, one iteration and the
-    LR-checked disparity, bit for bit (the oracle takes seconds per
-    iteration on the host cores)."""
-    L, R, _ = synth.make_pair(W, H, cells=24, dmax=dmax, channels=4, seed=0)
-    r = 4
-    print("full size", W, H, flush=True)
+    block row) and a gray 720p pair with the 11 x 11 patch at full size:
+    the initial fields, one iteration and the LR-checked disparity, bit
+    for bit (the oracle takes seconds per iteration on the host cores)."""
+    L, R, _ = synth.make_pair(W, H, cells=24, dmax=dmax, channels=C, seed=0)
+    print("full size", W, H, C, r, flush=True)
     ref = oracle.run(L, R, r, dmax, 1, maxdisp=1.0, maxdeg=180.0)
     with cvt_amd.Context(radius=r, disparity_max=dmax) as ctx:
         ctx.set_images(L, R)

@@ -57,6 +57,24 @@ def test_bilateral_weight_closed_form(oracle):
     assert np.array_equal(tex[0, 1, :3], img[0, 1].repeat(3))
 
 
+def test_bilateral_weight_follows_color_weight(oracle):
+    # colour factor 0: every tap weighs exp(-0) = 1 whatever the image, so
+    # the sum is the count of taps inside the image
+    rng = np.random.default_rng(3)
+    img = rng.uniform(0, 1, (9, 11, 4)).astype(np.float32)
+    tex = oracle.prepare(img, 2, color_weight=0.0)
+    ys, xs = np.mgrid[0:9, 0:11]
+    count = ((np.minimum(xs + 2, 10) - np.maximum(xs - 2, 0) + 1) *
+             (np.minimum(ys + 2, 8) - np.maximum(ys - 2, 0) + 1))
+    assert np.array_equal(tex[..., 3], count.astype(np.float32))
+    # factor 5 on the two-level image: exp(-5 |dot(delta, gray)|)
+    img = np.zeros((1, 3), np.float32)
+    img[0, 2] = 0.1
+    tex = oracle.prepare(img, 1, color_weight=5.0)
+    gsum = np.float32(0.2126) + np.float32(0.7152) + np.float32(0.0722)
+    assert abs(tex[0, 1, 3] - (2.0 + np.exp(-5 * 0.1 * gsum))) < 1e-6
+
+
 @pytest.mark.parametrize("C", [1, 4])
 def test_prepare_brute_force(oracle, C):
     rng = np.random.default_rng(10 + C)
