@@ -68,18 +68,23 @@ def test_rof_bitexact(oracle, pair, iters):
     assert_same(got, want, "rof")
 
 
-def test_huber_iterations_bitexact(oracle, pair):
-    L, R, _ = pair
-    r, dmax, n = 3, 20.0, 3
-    print("huber pipeline", flush=True)
-    with huber_ctx(r, dmax) as ctx:
+@pytest.mark.parametrize("which,r,dmax,n,tmin", [
+    ("rgba", 3, 20.0, 3, 1e-4),
+    ("gray", 5, 16.0, 3, 1e-4),     # 11 x 11 patch, CL_INTENSITY reads
+    ("rgba", 3, 20.0, 2, 0.05),     # a larger theta floor
+])
+def test_huber_iterations_bitexact(oracle, pair, pair_gray, which, r, dmax,
+                                   n, tmin):
+    L, R, _ = pair if which == "rgba" else pair_gray
+    print("huber pipeline", which, r, n, tmin, flush=True)
+    with huber_ctx(r, dmax, theta_min=tmin) as ctx:
         ctx.set_images(L, R)
         ctx.setup()
         ctx.iterate(n)
         sl, sr = ctx.planes(0), ctx.planes(1)
         ml, mr = ctx.smooth(0), ctx.smooth(1)
         d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)
-    ref = oracle.run_huber(L, R, r, dmax, n, 1e-4, 1.0, 5.0)
+    ref = oracle.run_huber(L, R, r, dmax, n, tmin, 1.0, 5.0)
     assert_same(sl, ref["stateL"], "left planes")
     assert_same(sr, ref["stateR"], "right planes")
     assert_same(ml, ref["smoothL"], "left smooth field")
