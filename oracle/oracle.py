@@ -66,16 +66,19 @@ _SIGS = {
 COLOR_WEIGHT = 12.0  # CENSUS_COLOR_WEIGHT, pmhstereo.cl:47
 
 
-def build():
-    """Compile the oracle with its Makefile (gcc, a few seconds)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+def build(always=True):
+    """Compile the oracle with its Makefile (gcc, a few seconds); with
+    always=False only when the library is missing or older than its
+    sources."""
+    cmd = ["make", "-s", "-C", _HERE] + (["-B"] if always else [])
+    subprocess.run(cmd, check=True)
 
 
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_SO):
-            build()
+            build(always=False)
         L = ctypes.CDLL(_SO)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)
