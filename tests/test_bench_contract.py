@@ -1,5 +1,6 @@
 """bench.py's contract that needs no GPU: it refuses more than one GPU
-with a message instead of measuring."""
+with a message instead of measuring, and reports profiled HBM traffic
+only for the kernel sources the counters were collected for."""
 import os
 import subprocess
 import sys
@@ -14,3 +15,25 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
     assert r.stdout.strip() == "" or not r.stdout.lstrip().startswith("{")
+
+
+def test_traffic_is_reported_only_for_the_profiled_sources(tmp_path):
+    """roofline.traffic comes from a profile file that records the hash of
+    the kernel sources its counters were collected for: a matching hash
+    gives the figure, an edited source or a missing file gives None."""
+    import hashlib
+    import json
+    sys.path.insert(0, ROOT)
+    import bench
+    src = tmp_path / "kernel.hip"
+    src.write_text("// kernel v1\n")
+    prof = tmp_path / "prof.json"
+    digest = hashlib.sha256(src.read_bytes()).hexdigest()
+    prof.write_text(json.dumps({"source_sha256": digest,
+                                "hbm_bytes_per_launch": 1.5e8}))
+    args = (str(prof), (str(src),), "hbm_bytes_per_launch")
+    assert bench.profiled_traffic(*args) == 1.5e8
+    src.write_text("// kernel v2\n")
+    assert bench.profiled_traffic(*args) is None
+    assert bench.profiled_traffic(str(tmp_path / "none.json"), (str(src),),
+                                  "hbm_bytes_per_launch") is None
