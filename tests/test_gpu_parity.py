@@ -584,3 +584,51 @@ def test_prior_init_full_size_bitexact(oracle, synth):
                 got = ctx.planes(v)
                 assert same(got, want[it][v]), (it, v,
                                                  first_diff(got, want[it][v]))
+
+
+def _random_configs(n, seed=2024):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        W = int(rng.integers(8, 200))
+        H = int(rng.integers(4, 120))
+        C = int(rng.choice([1, 4]))
+        r = int(rng.integers(1, 9))
+        dmax = float(rng.choice([4.0, 12.5, 31.0, 64.0]))
+        cw = float(rng.choice([0.0, 6.0, 12.0, 20.0]))
+        iters = int(rng.integers(1, 4))
+        out.append((W, H, C, r, dmax, cw, iters, int(rng.integers(0, 1000))))
+    return out
+
+
+@pytest.mark.parametrize("W,H,C,r,dmax,cw,iters,seed", _random_configs(12))
+def test_random_configurations_bitexact(oracle, synth, W, H, C, r, dmax, cw,
+                                        iters, seed):
+    """A seeded sweep over sizes (one or many block rows and columns,
+    partial work-groups), formats, radii, disparity ranges, colour weights
+    and iteration counts: the planes after every iteration and the
+    LR-checked disparity, bit for bit."""
+    L, R, _ = synth.make_pair(W, H, cells=3, dmax=min(dmax, W / 3.0),
+                              channels=C, seed=seed)
+    print("random config", W, H, C, r, dmax, cw, iters, seed, flush=True)
+    tl = oracle.prepare(L, r, color_weight=cw)
+    tr = oracle.prepare(R, r, color_weight=cw)
+    sl = oracle.init(tl, tr, r, dmax, 1, color_weight=cw)
+    sr = oracle.init(tr, tl, r, dmax, 0, color_weight=cw)
+    vb2 = oracle.viewbuf_empty(W, H)
+    with cvt_amd.Context(radius=r, disparity_max=dmax,
+                         color_weight=cw) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for it in range(iters):
+            sl, vb1 = oracle.propagate(tl, tr, r, dmax, 1, it, sl, vb2, sr,
+                                       color_weight=cw)
+            sr, vb2 = oracle.propagate(tr, tl, r, dmax, 0, it, sr, vb1, sl,
+                                       color_weight=cw)
+            ctx.iterate(1)
+            for view, want in ((0, sl), (1, sr)):
+                got = ctx.planes(view)
+                assert same(got, want), (it, view, first_diff(got, want))
+        d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)
+    want_d = oracle.to_disparity(oracle.consistency(sl, sr, 1.0, 5.0))
+    assert same(d, want_d), first_diff(d, want_d)
