@@ -182,3 +182,39 @@ def test_huber_full_size_bitexact(oracle, synth):
                "disp": ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)}
     for k, v in got.items():
         assert_same(v, ref[k], k)
+
+
+def _random_huber_configs(n, seed=77):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        W = int(rng.integers(12, 140))
+        H = int(rng.integers(6, 90))
+        C = int(rng.choice([1, 4]))
+        r = int(rng.integers(1, 7))
+        dmax = float(rng.choice([6.0, 16.0, 40.0]))
+        n_it = int(rng.integers(1, 4))
+        tmin = float(rng.choice([1e-4, 1e-2]))
+        out.append((W, H, C, r, dmax, n_it, tmin, int(rng.integers(0, 1000))))
+    return out
+
+
+@pytest.mark.parametrize("W,H,C,r,dmax,n,tmin,seed", _random_huber_configs(6))
+def test_random_huber_configurations_bitexact(oracle, synth, W, H, C, r, dmax,
+                                              n, tmin, seed):
+    """A seeded sweep of the coupled pipeline over sizes, formats, radii,
+    disparity ranges, iteration counts and theta floors: planes, smooth
+    fields and the disparity against oracle.run_huber, bit for bit."""
+    L, R, _ = synth.make_pair(W, H, cells=3, dmax=min(dmax, W / 3.0),
+                              channels=C, seed=seed)
+    print("random huber config", W, H, C, r, dmax, n, tmin, seed, flush=True)
+    ref = oracle.run_huber(L, R, r, dmax, n, tmin, 1.0, 5.0)
+    with huber_ctx(r, dmax, theta_min=tmin) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(n)
+        got = {"stateL": ctx.planes(0), "stateR": ctx.planes(1),
+               "smoothL": ctx.smooth(0), "smoothR": ctx.smooth(1),
+               "disp": ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)}
+    for k, v in got.items():
+        assert_same(v, ref[k], k)
