@@ -606,8 +606,8 @@ def test_random_configurations_bitexact(oracle, synth, W, H, C, r, dmax, cw,
                                         iters, seed):
     """A seeded sweep over sizes (one or many block rows and columns,
     partial work-groups), formats, radii, disparity ranges, colour weights
-    and iteration counts: the planes after every iteration and the
-    LR-checked disparity, bit for bit."""
+    and iteration counts: the planes after every iteration, the LR-checked
+    disparity, its hole-filled variant and the normals, bit for bit."""
     L, R, _ = synth.make_pair(W, H, cells=3, dmax=min(dmax, W / 3.0),
                               channels=C, seed=seed)
     print("random config", W, H, C, r, dmax, cw, iters, seed, flush=True)
@@ -630,5 +630,12 @@ def test_random_configurations_bitexact(oracle, synth, W, H, C, r, dmax, cw,
                 got = ctx.planes(view)
                 assert same(got, want), (it, view, first_diff(got, want))
         d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)
-    want_d = oracle.to_disparity(oracle.consistency(sl, sr, 1.0, 5.0))
+        df = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK_FILL, 1.0, 5.0)
+        nrm = ctx.normals(cvt_amd.DISPARITY_PP_LRCHECK_FILL)
+    chk = oracle.consistency(sl, sr, 1.0, 5.0)
+    want_d = oracle.to_disparity(chk)
     assert same(d, want_d), first_diff(d, want_d)
+    want_df = oracle.fill_depthmap(chk)
+    assert same(df, want_df), first_diff(df, want_df)
+    want_n = oracle.fill_normalmap(chk)
+    assert same(nrm, want_n), first_diff(nrm, want_n)
