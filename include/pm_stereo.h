@@ -130,9 +130,12 @@ int pm_setup_prior(pm_ctx* ctx, const float* disparity, const float* normals);
  * owns (each view's propagate runs as two halves; PM-Huber smoothing runs
  * beside the other view's propagate) and joins them back into the
  * context's stream before it returns, so the call stays ordered on that
- * stream like any other.  With PM_SMOOTH_NONE the call can be recorded by
- * stream capture on the context's stream; with PM_SMOOTH_HUBER a capturing
- * stream gets PM_ERR_ARG before anything is queued. */
+ * stream like any other.  Consecutive propagates overlap only within one
+ * call, so run the iterations as one call where possible (8 iterations of
+ * a 720p pair: 52.0 ms as one call, 56.9 ms as eight calls of one).  With
+ * PM_SMOOTH_NONE the call can be recorded by stream capture on the
+ * context's stream; with PM_SMOOTH_HUBER a capturing stream gets
+ * PM_ERR_ARG before anything is queued. */
 int pm_iterate(pm_ctx* ctx, int n);
 int pm_iteration(const pm_ctx* ctx);
 
