@@ -639,9 +639,14 @@ int postprocess(pm_ctx* c, int pp, float maxdispdiff, float maxdeg,
 // work-group fit the CU's LDS, allocated in 512-byte granules.
 int share_cus_with_smoothing(pm_ctx* c) {
   int lds_cu = 0;
-  PM_HIP(c, hipDeviceGetAttribute(
-                &lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
-                c->prm.device));
+  if (hipDeviceGetAttribute(&lds_cu,
+                            hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                            c->prm.device) != hipSuccess) {
+    // no floor: the propagate keeps the CUs to itself.  The failed query
+    // must not surface as the error of a later launch.
+    (void)hipGetLastError();
+    return PM_OK;
+  }
   const size_t floor = (size_t)lds_cu / 4 + 64;
   const size_t granule = 512;
   const size_t held = (floor + granule - 1) / granule * granule;
