@@ -896,8 +896,12 @@ k_math(const float* __restrict__ in, int n, float* __restrict__ out) {
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
+// Launches and returns the launch's own error.  The runtime's last-error
+// slot is cleared first: an earlier failed call, already returned to its
+// caller, is not reported again here.
 #define PM_LAUNCH(kernel, grid, block, lds, st, ...)                \
   do {                                                              \
+    (void)hipGetLastError();                                        \
     hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__);  \
     return hipGetLastError();                                       \
   } while (0)
