@@ -60,6 +60,12 @@ _SIGS = {
     "pmo_rof": (_i, [_p, _p, _p, _i, _i, _i, _p]),
     "pmo_run_huber": (_i, [_p, _p, _i, _i, _i, _i, _i, _f, _i, _f, _f, _f,
                            _p, _p, _p, _p, _p]),
+    "pmo_huber_step": (None, [_p, _p, _p, _p, _p, _p, _f, _i, _i, _p, _p,
+                              _p]),
+    "pmo_huber": (_i, [_p, _p, _p, _f, _i, _i, _i, _p]),
+    "pmo_fill_state": (None, [_p, _i, _i, _f, _i, _p]),
+    "pmo_run_depthmap": (_i, [_p, _p, _i, _i, _i, _i, _i, _f, _i, _f, _f,
+                              _p, _p, _p, _p, _p]),
     "pmo_num_threads": (_i, []),
 }
 
@@ -397,4 +403,54 @@ def run_huber(left, right, radius, dmax, iters, theta_min=HUBER_THETA_MIN,
                              _ptr(out["smoothR"]))
     if rc != 0:
         raise MemoryError("pmo_run_huber failed")
+    return out
+
+
+def huber_step(u, p, q, f, dt, lam, eps):
+    H, W = u.shape[:2]
+    uo = np.zeros((H, W, 4), np.float32)
+    po = np.zeros((H, W, 8), np.float32)
+    qo = np.zeros((H, W, 4), np.float32)
+    lam = np.asarray(lam, np.float32)
+    lib().pmo_huber_step(_ptr(_f32(u)), _ptr(_f32(p)), _ptr(_f32(q)),
+                         _ptr(_f32(f)), _ptr(_f32(dt)), _ptr(lam), float(eps),
+                         W, H, _ptr(uo), _ptr(po), _ptr(qo))
+    return uo, po, qo
+
+
+def huber(f, dt, lam, eps, iters=40):
+    H, W = f.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lam = np.asarray(lam, np.float32)
+    if lib().pmo_huber(_ptr(_f32(f)), _ptr(_f32(dt)), _ptr(lam), float(eps),
+                       int(iters), W, H, _ptr(out)):
+        raise ValueError("pmo_huber failed")
+    return out
+
+
+def fill_state(states, dispscale, lr):
+    H, W = states.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().pmo_fill_state(_ptr(_f32(states)), W, H, float(dispscale), int(lr),
+                         _ptr(out))
+    return out
+
+
+def run_depthmap(left, right, radius, dmax, iters, maxdisp=1.0, maxdeg=5.0):
+    """depthMap pipeline; returns dict of stateL, stateR, disp, smoothL,
+    smoothR (disparity at scale 1)."""
+    left, W, H, C, stride = _img_args(left)
+    right = np.ascontiguousarray(right, dtype=np.float32)
+    assert right.shape == left.shape
+    out = {k: np.zeros((H, W, 4), np.float32)
+           for k in ("stateL", "stateR", "smoothL", "smoothR")}
+    out["disp"] = np.zeros((H, W), np.float32)
+    rc = lib().pmo_run_depthmap(_ptr(left), _ptr(right), W, H, C, stride,
+                                int(radius), float(dmax), int(iters),
+                                float(maxdisp), float(maxdeg),
+                                _ptr(out["stateL"]), _ptr(out["stateR"]),
+                                _ptr(out["disp"]), _ptr(out["smoothL"]),
+                                _ptr(out["smoothR"]))
+    if rc != 0:
+        raise MemoryError("pmo_run_depthmap failed")
     return out

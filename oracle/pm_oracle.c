@@ -27,6 +27,7 @@
  */
 #include "pm_oracle.h"
 
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -1135,6 +1136,64 @@ void pmo_normal_depth(const float* states, int W, int H, float dispscale,
         }
 }
 
+/* float to int, truncating towards zero, with NaN -> 0 and values
+ * outside the int range saturated.  OpenCL leaves those cases undefined;
+ * the engine uses the same rule, so that host and device agree. */
+static int f2i_trunc(float v)
+{
+    if (v != v) return 0;
+    if (v >= 2147483648.0f) return INT_MAX;
+    if (v <= -2147483648.0f) return INT_MIN;
+    return (int)v;
+}
+
+/* pmhstereo_fill_state, pmhstereo.cl:1131-1185: a plane whose cost is
+ * negative or whose disparity lies outside [0, W] is replaced by the
+ * nearest plane to its left or right whose cost is not negative (the
+ * search tests only the cost); of two, the one with the smaller positive
+ * disparity at this pixel (<= 0 counts as W, ties to the left).  Then
+ * non-finite a, b become 0 and the plane is converted to (normal,
+ * disparity * dispscale) as pmhstereo_normal_depth does. */
+void pmo_fill_state(const float* states, int W, int H, float dispscale,
+                    int lr, float* out)
+{
+    const float sidesign = lr ? -1.0f : 1.0f;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; y++) {
+        const size_t row = (size_t)y * W;
+        for (int x = 0; x < W; x++) {
+            const float xf = (float)x, yf = (float)y;
+            f4 state = load4(states, row + x);
+            f4 val = state;
+            float disp = (transform(state, xf, yf) - xf) * sidesign;
+            if (state.w < 0.0f || disp < 0.0f || disp > (float)W) {
+                f4 left = mk4(-1.0f, -1.0f, -1.0f, -1.0f);
+                for (int i = x - 1; left.w < 0.0f && i >= 0; i--)
+                    left = load4(states, row + i);
+                f4 right = mk4(-1.0f, -1.0f, -1.0f, -1.0f);
+                for (int i = x + 1; right.w < 0.0f && i < W; i++)
+                    right = load4(states, row + i);
+                if (left.w >= 0.0f && right.w >= 0.0f) {
+                    int dl = f2i_trunc(sidesign * (transform(left, xf, yf) - xf));
+                    int dr = f2i_trunc(sidesign * (transform(right, xf, yf) - xf));
+                    dl = dl <= 0 ? W : dl;
+                    dr = dr <= 0 ? W : dr;
+                    val = dl <= dr ? left : right;
+                } else {
+                    val = right.w < 0.0f ? left : right;
+                }
+            }
+            if (!(isfinite(val.x) && isfinite(val.y))) {
+                val.x = 0.0f;
+                val.y = 0.0f;
+            }
+            val = to_ref_normal_depth(val, xf, yf, lr);
+            val.w = val.w * dispscale;
+            memcpy(out + (row + x) * 4, &val, sizeof val);
+        }
+    }
+}
+
 #define ROF_SIGMA 4.0f  /* PDDROFDiffTensor.cl:36-39 */
 #define ROF_TAU 0.05f
 #define ROF_ETA 8.0f
@@ -1142,9 +1201,12 @@ void pmo_normal_depth(const float* states, int W, int H, float dispscale,
 
 /* the tensor-applied projected dual at pixel (i, j) of one step
  * (PDDROFDiffTensor.cl:69-98): u read clamped to the edge, tensor and old
- * dual read as 0 outside the image */
-static void rof_dual(const float* u, const float* p, const float* dt, int W,
-                     int H, int i, int j, float lo[4], float hi[4])
+ * dual read as 0 outside the image.  eps >= 0 selects the Huber dual of
+ * PDDHuberDiffTensor.cl:64-92 instead: the ascent step divided by
+ * 1 + SIGMA eps, and one norm over all four channels (the ROF dual
+ * projects xyz jointly and w on its own). */
+static void pd_dual(const float* u, const float* p, const float* dt, int W,
+                    int H, int i, int j, float eps, float lo[4], float hi[4])
 {
     int inside = i >= 0 && i < W && j >= 0 && j < H;
     int ci = i < 0 ? 0 : (i > W - 1 ? W - 1 : i);
@@ -1163,17 +1225,28 @@ static void rof_dual(const float* u, const float* p, const float* dt, int W,
         memcpy(phi, p + q * 8 + 4, sizeof phi);
     }
     float n[4];
+    const float shrink = 1.0f + ROF_SIGMA * eps;
     for (int k = 0; k < 4; k++) {
         float dx = u0[k] - ux[k], dy = u0[k] - uy[k];
         float DTdx = D[0] * dx + D[2] * dy;
         float DTdy = D[1] * dx + D[3] * dy;
         plo[k] = plo[k] + ROF_SIGMA * DTdx;
         phi[k] = phi[k] + ROF_SIGMA * DTdy;
+        if (eps >= 0.0f) {
+            plo[k] = plo[k] / shrink;
+            phi[k] = phi[k] / shrink;
+        }
         n[k] = plo[k] * plo[k] + phi[k] * phi[k];
     }
-    float nxyz = sqrtf((n[0] + n[1]) + n[2]);
-    float proj[4] = {fmaxf(1.0f, nxyz), fmaxf(1.0f, nxyz), fmaxf(1.0f, nxyz),
-                     fmaxf(1.0f, sqrtf(n[3]))};
+    float proj[4];
+    if (eps >= 0.0f) {
+        float nall = fmaxf(1.0f, sqrtf(((n[0] + n[1]) + n[2]) + n[3]));
+        proj[0] = proj[1] = proj[2] = proj[3] = nall;
+    } else {
+        float nxyz = fmaxf(1.0f, sqrtf((n[0] + n[1]) + n[2]));
+        proj[0] = proj[1] = proj[2] = nxyz;
+        proj[3] = fmaxf(1.0f, sqrtf(n[3]));
+    }
     for (int k = 0; k < 4; k++) {
         plo[k] = plo[k] / proj[k];
         phi[k] = phi[k] / proj[k];
@@ -1182,19 +1255,21 @@ static void rof_dual(const float* u, const float* p, const float* dt, int W,
     }
 }
 
-/* PDDROFDiffTensor, PDDROFDiffTensor.cl:43-125 */
-void pmo_rof_step(const float* u, const float* p, const float* q,
-                  const float* f, const float* dt, const float lambda[4],
-                  int W, int H, float* u_out, float* p_out, float* q_out)
+/* PDDROFDiffTensor, PDDROFDiffTensor.cl:43-125, and with eps >= 0
+ * PDDHuberDiffTensor, PDDHuberDiffTensor.cl:38-109 (the same primal) */
+static void pd_step(const float* u, const float* p, const float* q,
+                    const float* f, const float* dt, const float lambda[4],
+                    float eps, int W, int H, float* u_out, float* p_out,
+                    float* q_out)
 {
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
             size_t o = (size_t)y * W + x;
             float Plo[4], Phi[4], Llo[4], Lhi[4], Ulo[4], Uhi[4];
-            rof_dual(u, p, dt, W, H, x, y, Plo, Phi);
-            rof_dual(u, p, dt, W, H, x - 1, y, Llo, Lhi);
-            rof_dual(u, p, dt, W, H, x, y - 1, Ulo, Uhi);
+            pd_dual(u, p, dt, W, H, x, y, eps, Plo, Phi);
+            pd_dual(u, p, dt, W, H, x - 1, y, eps, Llo, Lhi);
+            pd_dual(u, p, dt, W, H, x, y - 1, eps, Ulo, Uhi);
             for (int k = 0; k < 4; k++) {
                 float div = ((Plo[k] - Llo[k]) + Phi[k]) - Uhi[k];
                 float img = f[o * 4 + k], qq = q[o * 4 + k];
@@ -1213,11 +1288,28 @@ void pmo_rof_step(const float* u, const float* p, const float* q,
         }
 }
 
-/* denoiseDiffusionTensorROF_PDD, PDOptCL.cpp:1114-1168: output = input,
- * duals 0, `iters` steps (one more when the count is odd, so that the
- * result lands in the output image: never for the 40 the callers use) */
-int pmo_rof(const float* f, const float* dt, const float lambda[4],
-            int iters, int W, int H, float* u_out)
+void pmo_rof_step(const float* u, const float* p, const float* q,
+                  const float* f, const float* dt, const float lambda[4],
+                  int W, int H, float* u_out, float* p_out, float* q_out)
+{
+    pd_step(u, p, q, f, dt, lambda, -1.0f, W, H, u_out, p_out, q_out);
+}
+
+void pmo_huber_step(const float* u, const float* p, const float* q,
+                    const float* f, const float* dt, const float lambda[4],
+                    float eps, int W, int H, float* u_out, float* p_out,
+                    float* q_out)
+{
+    pd_step(u, p, q, f, dt, lambda, eps, W, H, u_out, p_out, q_out);
+}
+
+/* denoiseDiffusionTensorROF_PDD, PDOptCL.cpp:1114-1168, and with
+ * eps >= 0 denoiseDiffusionTensorHuber_PDD, PDOptCL.cpp:1170-1225 (the
+ * same host loop): output = input, duals 0, `iters` steps (one more when
+ * the count is odd, so that the result lands in the output image: never
+ * for the 40 the callers use) */
+static int pd_run(const float* f, const float* dt, const float lambda[4],
+                  float eps, int iters, int W, int H, float* u_out)
 {
     size_t n = (size_t)W * H;
     if (iters < 1) return -1;
@@ -1229,12 +1321,12 @@ int pmo_rof(const float* f, const float* dt, const float lambda[4],
     int swap = 0;
     for (int i = 0; i < iters; i++) {
         swap = i & 1;
-        pmo_rof_step(u[1 - swap], p[1 - swap], q[1 - swap], f, dt, lambda,
-                     W, H, u[swap], p[swap], q[swap]);
+        pd_step(u[1 - swap], p[1 - swap], q[1 - swap], f, dt, lambda, eps,
+                W, H, u[swap], p[swap], q[swap]);
     }
     if (!swap)
-        pmo_rof_step(u[0], p[0], q[0], f, dt, lambda, W, H, u[1], p[1],
-                     q[1]);
+        pd_step(u[0], p[0], q[0], f, dt, lambda, eps, W, H, u[1], p[1],
+                q[1]);
     memcpy(u_out, u[1], n * 16);
     for (int k = 0; k < 2; k++) {
         free(u[k]);
@@ -1242,6 +1334,19 @@ int pmo_rof(const float* f, const float* dt, const float lambda[4],
         free(q[k]);
     }
     return 0;
+}
+
+int pmo_rof(const float* f, const float* dt, const float lambda[4],
+            int iters, int W, int H, float* u_out)
+{
+    return pd_run(f, dt, lambda, -1.0f, iters, W, H, u_out);
+}
+
+int pmo_huber(const float* f, const float* dt, const float lambda[4],
+              float eps, int iters, int W, int H, float* u_out)
+{
+    if (!(eps >= 0.0f)) return -1;
+    return pd_run(f, dt, lambda, eps, iters, W, H, u_out);
 }
 
 int pmo_num_threads(void)
@@ -1363,6 +1468,91 @@ int pmo_run_huber(const float* left, const float* right, int W, int H,
         pmo_normal_depth(sR[1 - cur], W, H, 1.0f / dmax, 0, nd);
         if (pmo_rof(nd, dtR, lambda, 40, W, H, smR)) return -1;
         theta = powf((float)it / (float)iters, 4.0f) + theta_min;
+        cur = 1 - cur;
+    }
+    if (stateL) memcpy(stateL, sL[cur], n * 16);
+    if (stateR) memcpy(stateR, sR[cur], n * 16);
+    if (smoothL) memcpy(smoothL, smL, n * 16);
+    if (smoothR) memcpy(smoothR, smR, n * 16);
+    if (disp) {
+        pmo_consistency(sL[cur], sR[cur], W, H, maxdispdiff, maxdeg, chk);
+        pmo_to_disparity(chk, W, H, 1.0f, 1, disp);
+    }
+    free(texL);
+    free(texR);
+    free(sL[0]);
+    free(sL[1]);
+    free(sR[0]);
+    free(sR[1]);
+    free(dtL);
+    free(dtR);
+    free(smL);
+    free(smR);
+    free(nd);
+    free(vb1);
+    free(vb2);
+    free(chk);
+    return 0;
+}
+
+/* PMHuberStereo::depthMap (PMHuberStereo.cpp:147-347): the PM-Huber
+ * coupling with its own constants (:153-165): diffusion tensor alpha 12,
+ * beta 1; theta starts at 1e-3, the accept rule uses theta * 10, lambda
+ * is theta * 15 + 2 on all four channels; each view's smoothing input is
+ * its new field filled and converted by pmhstereo_fill_state (:273, :295)
+ * and smoothed by 40 diffusion-tensor Huber steps, eps 1e-4; after
+ * iteration i >= 3, theta = ((i - 3) / (N - 4))^3 + 1e-3 (:315-318).  The
+ * occlusion maps it computes feed only disabled code and are left out.
+ * Then the LR check and disparity conversion at scale 1. */
+int pmo_run_depthmap(const float* left, const float* right, int W, int H,
+                     int channels, int stride, int radius, float dmax,
+                     int iters, float maxdispdiff, float maxdeg,
+                     float* stateL, float* stateR, float* disp,
+                     float* smoothL, float* smoothR)
+{
+    const float cw = 12.0f;
+    size_t n = (size_t)W * H;
+    float* texL = (float*)malloc(n * 8 * sizeof(float));
+    float* texR = (float*)malloc(n * 8 * sizeof(float));
+    float* sL[2] = {(float*)malloc(n * 16), (float*)malloc(n * 16)};
+    float* sR[2] = {(float*)malloc(n * 16), (float*)malloc(n * 16)};
+    float* dtL = (float*)malloc(n * 16);
+    float* dtR = (float*)malloc(n * 16);
+    float* smL = (float*)calloc(n * 4, 4);
+    float* smR = (float*)calloc(n * 4, 4);
+    float* nd = (float*)malloc(n * 16);
+    uint64_t* vb1 = (uint64_t*)malloc(n * 8);
+    uint64_t* vb2 = (uint64_t*)malloc(n * 8);
+    float* chk = (float*)malloc(n * 16);
+    if (!texL || !texR || !sL[0] || !sL[1] || !sR[0] || !sR[1] || !dtL ||
+        !dtR || !smL || !smR || !nd || !vb1 || !vb2 || !chk)
+        return -1;
+    pmo_prepare(left, W, H, channels, stride, radius, cw, texL);
+    pmo_prepare(right, W, H, channels, stride, radius, cw, texR);
+    pmo_diffusion_tensor(left, W, H, channels, stride, 12.0f, dtL);
+    pmo_diffusion_tensor(right, W, H, channels, stride, 12.0f, dtR);
+    pmo_init(texL, texR, W, H, radius, cw, dmax, 1, sL[0]);
+    pmo_init(texR, texL, W, H, radius, cw, dmax, 0, sR[0]);
+    pmo_viewbuf_clear(vb2, W, H);
+    float theta = 1e-3f;
+    int cur = 0;
+    for (int it = 0; it < iters; it++) {
+        const float lam = theta * 15.0f + 2.0f;
+        const float lambda[4] = {lam, lam, lam, lam};
+        pmo_viewbuf_clear(vb1, W, H);
+        pmo_propagate(texL, texR, W, H, radius, cw, dmax, theta * 10.0f, 1,
+                      it, sL[cur], smL, vb2, sR[cur], vb1, sL[1 - cur]);
+        pmo_viewbuf_clear(vb2, W, H);
+        pmo_propagate(texR, texL, W, H, radius, cw, dmax, theta * 10.0f, 0,
+                      it, sR[cur], smR, vb1, sL[1 - cur], vb2, sR[1 - cur]);
+        pmo_fill_state(sL[1 - cur], W, H, 1.0f / dmax, 1, nd);
+        if (pmo_huber(nd, dtL, lambda, 1e-4f, 40, W, H, smL)) return -1;
+        pmo_fill_state(sR[1 - cur], W, H, 1.0f / dmax, 0, nd);
+        if (pmo_huber(nd, dtR, lambda, 1e-4f, 40, W, H, smR)) return -1;
+        if (it >= 3) {
+            theta = ((float)it - 3.0f) / ((float)iters - 4.0f);
+            theta = theta * theta * theta + 1e-3f;
+        }
         cur = 1 - cur;
     }
     if (stateL) memcpy(stateL, sL[cur], n * 16);

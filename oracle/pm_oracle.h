@@ -149,6 +149,23 @@ void pmo_rof_step(const float* u, const float* p, const float* q,
 int pmo_rof(const float* f, const float* dt, const float lambda[4],
             int iters, int W, int H, float* u_out);
 
+/* one PDDHuberDiffTensor step (cvt/cl/kernel/PDDHuberDiffTensor.cl:
+ * 38-109): the ROF step with the Huber dual (ascent divided by
+ * 1 + 4 eps, one norm over the four channels) */
+void pmo_huber_step(const float* u, const float* p, const float* q,
+                    const float* f, const float* dt, const float lambda[4],
+                    float eps, int W, int H, float* u_out, float* p_out,
+                    float* q_out);
+/* denoiseDiffusionTensorHuber_PDD (cvt/vision/PDOptCL.cpp:1170-1225),
+ * eps >= 0; u_out W x H x 4 */
+int pmo_huber(const float* f, const float* dt, const float lambda[4],
+              float eps, int iters, int W, int H, float* u_out);
+/* pmhstereo_fill_state (pmhstereo.cl:1131-1185): planes whose cost is
+ * negative or whose disparity is outside [0, W] replaced by a row
+ * neighbour, then (normal, disparity * dispscale), W x H x 4 */
+void pmo_fill_state(const float* states, int W, int H, float dispscale,
+                    int lr, float* out);
+
 /* ---- whole PM-Huber pipeline: disparityMapAll (PMHuberStereo.cpp:
  * 406-581) with one pyramid level: exactly `iters` iterations (the caller
  * makes the count odd as disparityMap does), theta from theta_min,
@@ -160,6 +177,15 @@ int pmo_run_huber(const float* left, const float* right, int W, int H,
                   int iters, float theta_min, float maxdispdiff,
                   float maxdeg, float* stateL, float* stateR, float* disp,
                   float* smoothL, float* smoothR);
+
+/* ---- the depthMap pipeline (PMHuberStereo.cpp:147-347): the coupling
+ * with diffusion-tensor Huber smoothing of filled fields, its own theta
+ * schedule and constants; outputs as pmo_run_huber. */
+int pmo_run_depthmap(const float* left, const float* right, int W, int H,
+                     int channels, int stride, int radius, float dmax,
+                     int iters, float maxdispdiff, float maxdeg,
+                     float* stateL, float* stateR, float* disp,
+                     float* smoothL, float* smoothR);
 
 int pmo_num_threads(void);
 

@@ -217,3 +217,141 @@ def test_huber_coupling_changes_the_result(oracle, synth):
     nd = O.normal_depth(hub["stateL"], 1.0 / 12.0, 1)
     ok = np.isfinite(nd).all(-1)
     assert np.median(np.abs(hub["smoothL"][..., 3] - nd[..., 3])[ok]) < 0.05
+
+
+
+# ---- the depthMap coupling: Huber dual, fill_state, pipeline wiring ------
+
+def _bits(a):
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).copy()
+    u[np.isnan(a)] = 0x7FC00000
+    return u
+
+
+def test_huber_step_with_eps_0_is_the_rof_step_when_unprojected(oracle):
+    # tiny variations: every dual stays inside the unit ball, so neither
+    # projection acts and the divisions by 1 + 4 * 0 are exact
+    rng = np.random.default_rng(5)
+    H, W = 9, 13
+    u = (0.5 + 1e-3 * rng.standard_normal((H, W, 4))).astype(np.float32)
+    f = (0.5 + 1e-3 * rng.standard_normal((H, W, 4))).astype(np.float32)
+    p = (1e-3 * rng.standard_normal((H, W, 8))).astype(np.float32)
+    q = (1e-3 * rng.standard_normal((H, W, 4))).astype(np.float32)
+    dt = np.zeros((H, W, 4), np.float32)
+    dt[..., 0] = 1.0
+    dt[..., 3] = 1.0
+    lam = [2.5, 2.5, 2.5, 2.5]
+    a = oracle.rof_step(u, p, q, f, dt, lam)
+    b = oracle.huber_step(u, p, q, f, dt, lam, 0.0)
+    for x, y in zip(a, b):
+        assert np.array_equal(_bits(x), _bits(y))
+
+
+def test_huber_dual_projects_all_four_channels_jointly(oracle):
+    # identity tensor: the stored dual is the projected one.  A step edge
+    # in every channel drives the ascent far outside the unit ball: the
+    # Huber dual ends on the 4-channel unit sphere, the ROF dual only on
+    # the xyz and w spheres separately (joint norm up to sqrt 2)
+    H, W = 6, 8
+    u = np.zeros((H, W, 4), np.float32)
+    u[:, W // 2:, :] = 1.0
+    dt = np.zeros((H, W, 4), np.float32)
+    dt[..., 0] = 1.0
+    dt[..., 3] = 1.0
+    z4, z8 = np.zeros((H, W, 4), np.float32), np.zeros((H, W, 8), np.float32)
+    lam = [2.0] * 4
+    _, ph, _ = oracle.huber_step(u, z8, z4, u, dt, lam, 1e-4)
+    _, pr, _ = oracle.rof_step(u, z8, z4, u, dt, lam)
+    joint = lambda p: np.sqrt((p.astype(np.float64) ** 2).sum(axis=-1))
+    assert joint(ph).max() <= 1.0 + 1e-6
+    assert abs(joint(ph).max() - 1.0) < 1e-6
+    assert joint(pr).max() > 1.3
+
+
+def test_fill_state_keeps_valid_planes(oracle):
+    # disparities inside [0, W] and costs >= 0: fill_state is
+    # normal_depth
+    rng = np.random.default_rng(2)
+    H, W = 5, 17
+    s = np.zeros((H, W, 4), np.float32)
+    s[..., 0] = 1.0 + 0.01 * rng.standard_normal((H, W))
+    s[..., 1] = 0.01 * rng.standard_normal((H, W))
+    s[..., 2] = -rng.uniform(1, 6, (H, W))
+    s[..., 3] = rng.uniform(0, 1, (H, W))
+    for lr, st in ((1, s), (0, s * np.float32([1, 1, -1, 1]))):
+        got = oracle.fill_state(st, 0.25, lr)
+        want = oracle.normal_depth(st, 0.25, lr)
+        assert np.array_equal(_bits(got), _bits(want))
+
+
+def _fronto(d):
+    # left-view plane of constant disparity d: x' = x - d
+    return np.float32([1.0, 0.0, -d, 0.5])
+
+
+def test_fill_state_takes_the_neighbour_with_the_smaller_disparity(oracle):
+    W = 6
+    s = np.stack([_fronto(3.0)] * W)[None].copy()
+    s[0, 2] = _fronto(-5.0)          # negative disparity: filled
+    s[0, 1] = _fronto(4.0)           # left neighbour, disparity 4
+    s[0, 3] = _fronto(2.0)           # right neighbour, disparity 2
+    out = oracle.fill_state(s, 1.0, 1)
+    want = oracle.normal_depth(s[:, 3:4], 1.0, 1)[0, 0]
+    # the right neighbour's plane evaluated at x = 2
+    nd = oracle.normal_depth(np.stack([s[0, 3]] * W)[None], 1.0, 1)
+    assert np.array_equal(_bits(out[0, 2]), _bits(nd[0, 2]))
+    s[0, 1] = _fronto(1.0)           # now the left one is nearer
+    out = oracle.fill_state(s, 1.0, 1)
+    nd = oracle.normal_depth(np.stack([s[0, 1]] * W)[None], 1.0, 1)
+    assert np.array_equal(_bits(out[0, 2]), _bits(nd[0, 2]))
+    # the left edge has no left neighbour: the right one is taken
+    s[0, 0] = _fronto(-2.0)
+    out = oracle.fill_state(s, 1.0, 1)
+    nd = oracle.normal_depth(np.stack([s[0, 1]] * W)[None], 1.0, 1)
+    assert np.array_equal(_bits(out[0, 0]), _bits(nd[0, 0]))
+    assert want.shape == (4,)
+
+
+def test_fill_state_zeroes_non_finite_slopes(oracle):
+    s = np.stack([_fronto(2.0)] * 4)[None].copy()
+    s[0, 1, 0] = np.nan                # a NaN plane keeps its place ...
+    out = oracle.fill_state(s, 1.0, 1)
+    t = s.copy()
+    t[0, 1, 0] = 0.0                   # ... with a, b set to 0
+    t[0, 1, 1] = 0.0
+    want = oracle.normal_depth(t, 1.0, 1)
+    assert np.array_equal(_bits(out[0, 1]), _bits(want[0, 1]))
+
+
+def test_run_depthmap_composes_its_pieces(oracle, synth):
+    """Two iterations of the pipeline against the same sequence built from
+    the oracle's pinned pieces: tensor alpha 12, theta 1e-3 (accept rule
+    theta * 10, lambda theta * 15 + 2), fill_state, 40 Huber steps with
+    eps 1e-4."""
+    L, R, _ = synth.make_pair(40, 24, cells=3, dmax=8.0, channels=4, seed=4)
+    r, dmax, n = 2, 8.0, 2
+    ref = oracle.run_depthmap(L, R, r, dmax, n, 1.0, 5.0)
+    tl, tr = oracle.prepare(L, r), oracle.prepare(R, r)
+    dtl = oracle.diffusion_tensor(L, 12.0)
+    dtr = oracle.diffusion_tensor(R, 12.0)
+    sl = oracle.init(tl, tr, r, dmax, 1)
+    sr = oracle.init(tr, tl, r, dmax, 0)
+    sml = np.zeros_like(sl)
+    smr = np.zeros_like(sr)
+    vb2 = oracle.viewbuf_empty(40, 24)
+    theta = np.float32(1e-3)
+    for it in range(n):
+        lam = np.float32(theta * np.float32(15.0) + np.float32(2.0))
+        th = np.float32(theta * np.float32(10.0))
+        sl, vb1 = oracle.propagate(tl, tr, r, dmax, 1, it, sl, vb2, sr,
+                                   smooth=sml, theta=th)
+        sr, vb2 = oracle.propagate(tr, tl, r, dmax, 0, it, sr, vb1, sl,
+                                   smooth=smr, theta=th)
+        sml = oracle.huber(oracle.fill_state(sl, np.float32(1.0) / np.float32(dmax), 1),
+                           dtl, [lam] * 4, 1e-4)
+        smr = oracle.huber(oracle.fill_state(sr, np.float32(1.0) / np.float32(dmax), 0),
+                           dtr, [lam] * 4, 1e-4)
+    for k, v in (("stateL", sl), ("stateR", sr), ("smoothL", sml),
+                 ("smoothR", smr)):
+        assert np.array_equal(_bits(ref[k]), _bits(v)), k
