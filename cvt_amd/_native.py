@@ -12,7 +12,7 @@ SO_PATH = os.path.join(HERE, "libcvt_amd_pm.so")
 
 PM_OK, PM_ERR_ARG, PM_ERR_STATE, PM_ERR_HIP, PM_ERR_NOMEM, PM_ERR_IO = range(6)
 PM_PP_NONE, PM_PP_LRCHECK, PM_PP_LRCHECK_FILL = range(3)
-PM_SMOOTH_NONE, PM_SMOOTH_HUBER = range(2)
+PM_SMOOTH_NONE, PM_SMOOTH_HUBER, PM_SMOOTH_DEPTHMAP = range(3)
 
 _p = ctypes.c_void_p
 _i = ctypes.c_int

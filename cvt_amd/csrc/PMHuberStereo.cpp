@@ -5,7 +5,7 @@
 namespace cvt_amd {
 
 PMHuberStereo::PMHuberStereo(int device, pm_smoothing smoothing)
-    : device_(device), smoothing_(smoothing) {}
+    : device_(device), smoothing_(smoothing), mode_(smoothing) {}
 
 PMHuberStereo::~PMHuberStereo() {
   if (ctx_) pm_destroy(ctx_);
@@ -18,7 +18,9 @@ void PMHuberStereo::check(int rc) const {
 
 // (re)create the context when the patch size or disparity range change
 void PMHuberStereo::configure(size_t patchsize, float depthmax) {
-  if (ctx_ && (int)patchsize == radius_ && depthmax == dmax_) return;
+  if (ctx_ && (int)patchsize == radius_ && depthmax == dmax_ &&
+      mode_ == ctx_mode_)
+    return;
   if (ctx_) {
     pm_destroy(ctx_);
     ctx_ = nullptr;
@@ -27,7 +29,7 @@ void PMHuberStereo::configure(size_t patchsize, float depthmax) {
   pm_params_default(&p);
   p.radius = (int)patchsize;
   p.disparity_max = depthmax;
-  p.smoothing = smoothing_;
+  p.smoothing = mode_;
   p.device = device_;
   pm_ctx* c = nullptr;
   int rc = pm_create(&p, &c);
@@ -37,6 +39,7 @@ void PMHuberStereo::configure(size_t patchsize, float depthmax) {
     throw Exception(rc, "PMHuberStereo: " + msg);
   }
   ctx_ = c;
+  ctx_mode_ = mode_;
   radius_ = (int)patchsize;
   dmax_ = depthmax;
 }
@@ -131,9 +134,19 @@ void PMHuberStereo::depthMap(Image& dmap, const Image& left,
                              DisparityPostProcessing pp, Image* normalmap,
                              const Image* initdisparity,
                              const Image* initnormalmap) {
-  if (dscale <= 0.0f) dscale = 1.0f / depthmax;  // PMHuberStereo.cpp:186-187
-  disparityMap(dmap, left, right, patchsize, depthmax, iterations, pp,
-               normalmap, initdisparity, initnormalmap);
+  if (dscale <= 0.0f) dscale = 1.0f / depthmax;  // PMHuberStereo.cpp:168-169
+  const float maxdispdiff = 1.0f;   // PMHuberStereo.cpp:156
+  const float maxanglediff = 5.0f;  // PMHuberStereo.cpp:157
+  mode_ = PM_SMOOTH_DEPTHMAP;
+  try {
+    setup(left, right, patchsize, depthmax, initdisparity, initnormalmap);
+    iterate(iterations);  // no `| 1` here (PMHuberStereo.cpp:214)
+    disparity(dmap, pp, normalmap, maxdispdiff, maxanglediff);
+  } catch (...) {
+    mode_ = smoothing_;
+    throw;
+  }
+  mode_ = smoothing_;
   for (size_t y = 0; y < dmap.height(); ++y)
     for (size_t x = 0; x < dmap.width(); ++x) dmap.at(x, y) *= dscale;
 }

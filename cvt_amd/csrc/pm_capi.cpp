@@ -156,7 +156,8 @@ int check_params(pm_ctx* c, const pm_params* p) {
     return fail(c, PM_ERR_ARG, "disparity_max must be finite and > 0");
   if (!(p->color_weight >= 0.0f) || !std::isfinite(p->color_weight))
     return fail(c, PM_ERR_ARG, "color_weight must be finite and >= 0");
-  if (p->smoothing != PM_SMOOTH_NONE && p->smoothing != PM_SMOOTH_HUBER)
+  if (p->smoothing != PM_SMOOTH_NONE && p->smoothing != PM_SMOOTH_HUBER &&
+      p->smoothing != PM_SMOOTH_DEPTHMAP)
     return fail(c, PM_ERR_ARG, "unknown smoothing %d", p->smoothing);
   if (!(p->theta_min >= 0.0f) || !std::isfinite(p->theta_min))
     return fail(c, PM_ERR_ARG, "theta_min must be finite and >= 0");
@@ -192,7 +193,7 @@ int alloc_pair(pm_ctx* c, int W, int H, int channels) {
   if ((rc = ensure(c, c->scratch4, n * sizeof(float4)))) return rc;
   if ((rc = ensure(c, c->scratch4b, n * sizeof(float4)))) return rc;
   if ((rc = ensure(c, c->scratch1, n * sizeof(float)))) return rc;
-  if (c->prm.smoothing == PM_SMOOTH_HUBER) {
+  if (c->prm.smoothing != PM_SMOOTH_NONE) {
     for (int v = 0; v < 2; ++v) {
       if ((rc = ensure(c, c->dt[v], n * sizeof(float4)))) return rc;
       if ((rc = ensure(c, c->smooth[v], n * sizeof(float4)))) return rc;
@@ -316,6 +317,17 @@ constexpr float kThetaScaleNorm = 10.0f;  // THETASCALE_NORM = THETASCALE_D
 constexpr float kThetaBias = 1.0f;        // THETABIAS_NORM = THETABIAS_D
 constexpr float kThetaScale = 50.0f;      // THETASCALE
 constexpr int kRofIters = 40;             // PDOPTITER
+// depthMap's constants (PMHuberStereo.cpp:153-165)
+constexpr float kDepthAlpha = 12.0f;      // diffusion tensor, beta = 1
+constexpr float kDepthThetaScaleD = 15.0f;
+constexpr float kDepthThetaBiasD = 2.0f;
+constexpr float kDepthThetaScale = 10.0f;
+constexpr float kDepthHuberEps = 1e-4f;   // HUBEREPS
+constexpr float kDepthTheta0 = 1e-3f;
+
+bool depthmap_mode(const pm_ctx* c) {
+  return c->prm.smoothing == PM_SMOOTH_DEPTHMAP;
+}
 
 // LAMBDA (PMHuberStereo.cpp:532): the normal channels and nz take
 // theta * 10 + 1, the depth channel 1
@@ -329,7 +341,7 @@ float4 huber_lambda(float theta) {
 // when the count is odd, so that the result lands in out).  f must not
 // alias out.
 int rof(pm_ctx* c, int view, const float4* f, float4 lambda, int iters,
-        float4* out, hipStream_t st) {
+        float4* out, hipStream_t st, float hubereps = -1.0f) {
   const size_t n = c->geo.npix;
   pm_ctx::RofBufs& b = c->rofb[view];
   float4* u[2] = {f4buf(b.u), out};
@@ -352,8 +364,8 @@ int rof(pm_ctx* c, int view, const float4* f, float4 lambda, int iters,
   for (int done = 0; done < steps; done += pm::kRofStepsMax) {
     const int k = std::min(pm::kRofStepsMax, steps - done);
     PM_HIP(c, pm::launch_rof_steps(st, c->geo, k, u[src], p[src],
-                                   q[src], f, dt, lambda, u[1 - src],
-                                   p[1 - src], q[1 - src]));
+                                   q[src], f, dt, lambda, hubereps,
+                                   u[1 - src], p[1 - src], q[1 - src]));
     src = 1 - src;
   }
   return PM_OK;
@@ -364,6 +376,16 @@ int rof(pm_ctx* c, int view, const float4* f, float4 lambda, int iters,
 int smooth_view(pm_ctx* c, int view, const float4* planes, float theta,
                 hipStream_t st) {
   float4* nd = f4buf(c->rofb[view].nd);
+  if (depthmap_mode(c)) {
+    // depthMap: filled field, Huber steps, lambda theta * 15 + 2 on all
+    // four channels (PMHuberStereo.cpp:249, 273-274, 295-296)
+    PM_HIP(c, pm::launch_fill_state(st, c->geo, planes,
+                                    1.0f / c->prm.disparity_max,
+                                    view == 0 ? 1 : 0, nd));
+    const float l = theta * kDepthThetaScaleD + kDepthThetaBiasD;
+    return rof(c, view, nd, make_float4(l, l, l, l), kRofIters,
+               f4buf(c->smooth[view]), st, kDepthHuberEps);
+  }
   PM_HIP(c, pm::launch_normal_depth(st, c->geo, planes,
                                     1.0f / c->prm.disparity_max,
                                     view == 0 ? 1 : 0, nd));
@@ -444,7 +466,7 @@ int propagate_split(pm_ctx* c, const pm::PropagateArgs& a, int view,
       if (j != k && s0 < r1 + kBandReach && s1 > r0 - kBandReach)
         PM_HIP(c, hipStreamWaitEvent(band_stream(c, k), ev[j], 0));
     }
-    if (c->prm.smoothing == PM_SMOOTH_HUBER)  // the view's smooth field
+    if (c->prm.smoothing != PM_SMOOTH_NONE)  // the view's smooth field
       PM_HIP(c, hipStreamWaitEvent(band_stream(c, k), c->ev_smooth[view], 0));
   }
   for (int k = 0; k < kSplitParts; ++k) {
@@ -490,17 +512,31 @@ int smooth_after(pm_ctx* c, int view, const float4* planes, float theta,
   return PM_OK;
 }
 
+// theta after iteration i (0-based) of a schedule of c->schedule_n:
+// disparityMapAll's (i / N)^4 + theta_min (PMHuberStereo.cpp:575-576), or
+// depthMap's ((i - 3) / (N - 4))^3 + 1e-3 from i = 3 on, unchanged before
+// (PMHuberStereo.cpp:315-318)
+float theta_after(const pm_ctx* c, int i, float theta) {
+  if (depthmap_mode(c)) {
+    if (i < 3) return theta;
+    const float t = ((float)i - 3.0f) / ((float)c->schedule_n - 4.0f);
+    return t * t * t + kDepthTheta0;
+  }
+  return std::pow((float)i / (float)c->schedule_n, 4.0f) + c->prm.theta_min;
+}
+
 // pmhstereo_propagate_view for the left then the right view, each followed
 // by its PM-Huber smoothing (PMHuberStereo.cpp:530-578)
 int one_iteration(pm_ctx* c, unsigned long long* counters, bool split) {
   const int cur = c->cur, nxt = 1 - c->cur;
   unsigned long long* vbL = static_cast<unsigned long long*>(c->vbuf[0].ptr);
   unsigned long long* vbR = static_cast<unsigned long long*>(c->vbuf[1].ptr);
-  const bool huber = c->prm.smoothing == PM_SMOOTH_HUBER;
+  const bool huber = c->prm.smoothing != PM_SMOOTH_NONE;
   const float theta = c->theta;
   pm::PropagateArgs a;
   a.dmax = c->prm.disparity_max;
-  a.theta = huber ? theta * kThetaScale : 0.0f;
+  a.theta = huber ? theta * (depthmap_mode(c) ? kDepthThetaScale : kThetaScale)
+                 : 0.0f;
   a.iter = c->iteration;
   int rc;
 
@@ -531,9 +567,7 @@ int one_iteration(pm_ctx* c, unsigned long long* counters, bool split) {
     return rc;
 
   if (huber) {
-    // theta = (iter / N)^4 + theta_min (PMHuberStereo.cpp:575-576)
-    c->theta = std::pow((float)c->iteration / (float)c->schedule_n, 4.0f) +
-               c->prm.theta_min;
+    c->theta = theta_after(c, c->iteration, c->theta);
   }
   c->cur = nxt;
   c->iteration += 1;
@@ -545,15 +579,15 @@ int one_iteration(pm_ctx* c, unsigned long long* counters, bool split) {
 int iterate_n(pm_ctx* c, int n, unsigned long long* counters) {
   const bool split = n > 0 && can_split(c);
   int rc;
-  if (c->prm.smoothing == PM_SMOOTH_HUBER) {
+  if (c->prm.smoothing != PM_SMOOTH_NONE) {
     // Capturing the PM-Huber iteration into a graph ended with the process
     // crashing inside the runtime's end of capture (plain PatchMatch
     // captures and replays correctly); refused before anything is queued.
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     PM_HIP(c, hipStreamIsCapturing(c->stream, &cs));
     if (cs != hipStreamCaptureStatusNone)
-      return fail(c, PM_ERR_ARG, "pm_iterate with PM_SMOOTH_HUBER cannot "
-                  "be recorded by stream capture");
+      return fail(c, PM_ERR_ARG, "pm_iterate with smoothing cannot be "
+                  "recorded by stream capture");
   }
   if (split && (rc = split_rebase(c))) return rc;
   for (int i = 0; i < n; ++i)
@@ -565,7 +599,7 @@ int iterate_n(pm_ctx* c, int n, unsigned long long* counters) {
     // by the next split_rebase).  The smoothing stream joins only when it
     // ran something: otherwise it never waited for the context's stream,
     // and under stream capture its record would lie outside the capture.
-    const bool huber = c->prm.smoothing == PM_SMOOTH_HUBER;
+    const bool huber = c->prm.smoothing != PM_SMOOTH_NONE;
     for (int k = huber ? 0 : 1; k < kSplitParts; ++k) {
       const hipStream_t st = k == 0 ? c->smoothing : c->side[k - 1];
       hipEvent_t ev = k == 0 ? c->ev_smooth[1] : c->ev_band[1][k];
@@ -586,16 +620,19 @@ void start_schedule(pm_ctx* c, int n) {
 }
 
 // PM-Huber part of the setup: diffusion tensors of both images, cleared
-// smooth fields, theta = theta_min (PMHuberStereo.cpp:452-458)
+// smooth fields, theta = theta_min (PMHuberStereo.cpp:452-458); depthMap:
+// tensor alpha 12, theta 1e-3 (PMHuberStereo.cpp:153, 166, 190-195)
 int huber_setup(pm_ctx* c) {
-  c->theta = c->prm.theta_min;
-  if (c->prm.smoothing != PM_SMOOTH_HUBER) return PM_OK;
+  c->theta = depthmap_mode(c) ? kDepthTheta0 : c->prm.theta_min;
+  if (c->prm.smoothing == PM_SMOOTH_NONE) return PM_OK;
   const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   for (int v = 0; v < 2; ++v) {
     PM_HIP(c, pm::launch_diff_tensor(c->stream, c->geo,
                                      static_cast<const float*>(c->img[v].ptr),
                                      c->channels, c->geo.W * c->channels,
-                                     kHuberAlpha, f4buf(c->dt[v])));
+                                     depthmap_mode(c) ? kDepthAlpha
+                                                      : kHuberAlpha,
+                                     f4buf(c->dt[v])));
     PM_HIP(c, pm::launch_fill4(c->stream, c->geo.npix, zero,
                                f4buf(c->smooth[v])));
   }
@@ -695,7 +732,7 @@ int pm_create(const pm_params* p, pm_ctx** out) {
   }
   *out = c;
   if ((rc = use_device(c))) return rc;
-  if (p->smoothing == PM_SMOOTH_HUBER && (rc = share_cus_with_smoothing(c)))
+  if (p->smoothing != PM_SMOOTH_NONE && (rc = share_cus_with_smoothing(c)))
     return rc;
   if (p->stream) {
     c->stream = static_cast<hipStream_t>(p->stream);
@@ -892,13 +929,12 @@ int pm_set_planes(pm_ctx* c, int view, const float* in, int iteration) {
   c->iteration = iteration;
   // PM-Huber: theta of the next iteration when the schedule length is
   // fixed (PMHuberStereo.cpp:575-576); the smooth fields stay as they are
-  if (c->prm.smoothing == PM_SMOOTH_HUBER && c->prm.schedule_iterations > 0)
-    c->theta = iteration == 0
-                   ? c->prm.theta_min
-                   : std::pow((float)(iteration - 1) /
-                                  (float)c->prm.schedule_iterations,
-                              4.0f) +
-                         c->prm.theta_min;
+  if (c->prm.smoothing != PM_SMOOTH_NONE && c->prm.schedule_iterations > 0) {
+    c->schedule_n = c->prm.schedule_iterations;
+    float theta = depthmap_mode(c) ? kDepthTheta0 : c->prm.theta_min;
+    for (int i = 0; i < iteration; ++i) theta = theta_after(c, i, theta);
+    c->theta = theta;
+  }
   c->last_pp = -1;
   return PM_OK;
 }
@@ -1053,8 +1089,8 @@ int pm_copy_probe(pm_ctx* c, const float* d_src, float* d_dst, size_t bytes) {
 }
 
 static int require_huber(pm_ctx* c) {
-  if (c->prm.smoothing != PM_SMOOTH_HUBER)
-    return fail(c, PM_ERR_STATE, "the context does not use PM_SMOOTH_HUBER");
+  if (c->prm.smoothing == PM_SMOOTH_NONE)
+    return fail(c, PM_ERR_STATE, "the context does not smooth (PM_SMOOTH_NONE)");
   return PM_OK;
 }
 

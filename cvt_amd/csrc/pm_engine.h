@@ -131,10 +131,18 @@ hipError_t launch_normal_depth(hipStream_t st, const Geometry& g,
 constexpr int kRofStepsMax = 2;
 // LDS of one k_rof_steps work-group (bytes)
 size_t rof_steps_lds_bytes();
+// hubereps >= 0: diffusion-tensor Huber steps with that epsilon instead
+// (PDDHuberDiffTensor.cl:38-109)
 hipError_t launch_rof_steps(hipStream_t st, const Geometry& g, int nsteps,
                             const float4* u, const float4* p, const float4* q,
                             const float4* f, const float4* dt, float4 lambda,
-                            float4* u_out, float4* p_out, float4* q_out);
+                            float hubereps, float4* u_out, float4* p_out,
+                            float4* q_out);
+
+// pmhstereo_fill_state: filled (normal, disparity * dispscale) field
+hipError_t launch_fill_state(hipStream_t st, const Geometry& g,
+                             const float4* states, float dispscale, int lr,
+                             float4* out);
 
 hipError_t launch_fill4(hipStream_t st, size_t n, float4 v, float4* out);
 

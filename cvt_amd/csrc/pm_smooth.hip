@@ -5,6 +5,8 @@
 // 64 x 4 work-groups; every step is a bandwidth-bound stencil.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "pm_device.h"
 #include "pm_engine.h"
 
@@ -59,6 +61,59 @@ k_normal_depth(int W, int H, const float4* __restrict__ states,
   out[p] = v;
 }
 
+// float to int, truncating towards zero, with NaN -> 0 and values
+// outside the int range saturated (the conversion OpenCL leaves undefined
+// there, fixed so that every device and the host agree)
+__device__ __forceinline__ int f2i_trunc(float v) {
+  if (v != v) return 0;
+  if (v >= 2147483648.0f) return INT_MAX;
+  if (v <= -2147483648.0f) return INT_MIN;
+  return (int)v;
+}
+
+// pmhstereo_fill_state (pmhstereo.cl:1131-1185): a plane whose cost is
+// negative or whose disparity lies outside [0, W] is replaced by the
+// nearest plane to its left or right whose cost is not negative (the
+// search tests only the cost); of two, the one with the smaller positive
+// disparity at this pixel (<= 0 counts as W, ties to the left).  Then
+// non-finite a, b become 0 and the plane is converted as k_normal_depth
+// converts it.
+__global__ void __launch_bounds__(256)
+k_fill_state(int W, int H, const float4* __restrict__ states,
+             float dispscale, int lr, float4* __restrict__ out) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  const float4* row = states + (size_t)y * W;
+  const float xf = (float)x, yf = (float)y;
+  const float sidesign = lr ? -1.0f : 1.0f;
+  const float4 state = row[x];
+  float4 val = state;
+  const float disp = (transform(state, xf, yf) - xf) * sidesign;
+  if (state.w < 0.0f || disp < 0.0f || disp > (float)W) {
+    float4 left = mk4(-1.0f, -1.0f, -1.0f, -1.0f);
+    for (int i = x - 1; left.w < 0.0f && i >= 0; --i) left = row[i];
+    float4 right = mk4(-1.0f, -1.0f, -1.0f, -1.0f);
+    for (int i = x + 1; right.w < 0.0f && i < W; ++i) right = row[i];
+    if (left.w >= 0.0f && right.w >= 0.0f) {
+      int dl = f2i_trunc(sidesign * (transform(left, xf, yf) - xf));
+      int dr = f2i_trunc(sidesign * (transform(right, xf, yf) - xf));
+      dl = dl <= 0 ? W : dl;
+      dr = dr <= 0 ? W : dr;
+      val = dl <= dr ? left : right;
+    } else {
+      val = right.w < 0.0f ? left : right;
+    }
+  }
+  if (!(finite_f(val.x) && finite_f(val.y))) {
+    val.x = 0.0f;
+    val.y = 0.0f;
+  }
+  val = to_ref_normal_depth(val, xf, yf, lr);
+  val.w = val.w * dispscale;
+  out[(size_t)y * W + x] = val;
+}
+
 constexpr float kRofSigma = 4.0f;  // PDDROFDiffTensor.cl:36-39
 constexpr float kRofTau = 0.05f;
 constexpr float kRofEta = 8.0f;
@@ -75,10 +130,13 @@ struct Dual {
 // tensor-applied projected dual of one step (PDDROFDiffTensor.cl:69-98)
 // from u at the pixel, its right and its lower neighbour (clamped to the
 // edge), the tensor D and the old dual (both 0 outside the image)
+// kHuber: the dual of PDDHuberDiffTensor.cl:64-92 instead, the ascent
+// divided by shrink = 1 + SIGMA eps and one norm over all four channels
+template <bool kHuber>
 __device__ __forceinline__ Dual dual_of(const float4& u0, const float4& ux,
                                         const float4& uy, const float4& D,
                                         const float4& plo4,
-                                        const float4& phi4) {
+                                        const float4& phi4, float shrink) {
   Dual r;
   float n[4];
 #pragma unroll
@@ -88,10 +146,16 @@ __device__ __forceinline__ Dual dual_of(const float4& u0, const float4& ux,
     const float DTdy = D.y * dx + D.w * dy;
     r.lo[k] = comp(plo4, k) + kRofSigma * DTdx;
     r.hi[k] = comp(phi4, k) + kRofSigma * DTdy;
+    if (kHuber) {
+      r.lo[k] = r.lo[k] / shrink;
+      r.hi[k] = r.hi[k] / shrink;
+    }
     n[k] = r.lo[k] * r.lo[k] + r.hi[k] * r.hi[k];
   }
-  const float pxyz = fmaxf(1.0f, sqrtf((n[0] + n[1]) + n[2]));
-  const float pw = fmaxf(1.0f, sqrtf(n[3]));
+  const float pxyz =
+      kHuber ? fmaxf(1.0f, sqrtf(((n[0] + n[1]) + n[2]) + n[3]))
+             : fmaxf(1.0f, sqrtf((n[0] + n[1]) + n[2]));
+  const float pw = kHuber ? pxyz : fmaxf(1.0f, sqrtf(n[3]));
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const float pr = k < 3 ? pxyz : pw;
@@ -154,14 +218,16 @@ constexpr int kRofRH = kRofTY + 2 * kRofK;
 constexpr int kRofRows = kRofRH / 4;  // region rows per wavefront
 static_assert(kRofRH % 4 == 0, "four wavefronts share the region rows");
 
+template <bool kHuber>
 __global__ void __launch_bounds__(256)
-k_rof_steps(int W, int H, int nsteps, const float4* __restrict__ u,
+k_rof_steps(int W, int H, int nsteps, float eps, const float4* __restrict__ u,
             const float4* __restrict__ p, const float4* __restrict__ q,
             const float4* __restrict__ f, const float4* __restrict__ dt,
             float4 lambda, float4* __restrict__ u_out,
             float4* __restrict__ p_out, float4* __restrict__ q_out) {
   __shared__ float4 su[kRofRH * kRofRW];
   __shared__ float4 sp[2 * kRofRH * kRofRW];
+  const float shrink = 1.0f + kRofSigma * eps;  // Huber dual only
   const int lx = threadIdx.x;
   const int rx0 = blockIdx.x * kRofTX - kRofK;
   const int ry0 = blockIdx.y * kRofTY - kRofK;
@@ -200,8 +266,9 @@ k_rof_steps(int W, int H, int nsteps, const float4* __restrict__ u,
       const int cj1 = (clampi(gy + 1, 0, H - 1) - ry0) * kRofRW;
       const int l = ly * kRofRW + lx;
       const Dual P =
-          dual_of(su[cj + ci], su[cj + ci1], su[cj1 + ci], dr[r],
-                  inside ? sp[2 * l] : zero, inside ? sp[2 * l + 1] : zero);
+          dual_of<kHuber>(su[cj + ci], su[cj + ci1], su[cj1 + ci], dr[r],
+                          inside ? sp[2 * l] : zero,
+                          inside ? sp[2 * l + 1] : zero, shrink);
       sp[2 * l] = mk4(P.lo[0], P.lo[1], P.lo[2], P.lo[3]);
       sp[2 * l + 1] = mk4(P.hi[0], P.hi[1], P.hi[2], P.hi[3]);
     }
@@ -288,12 +355,23 @@ size_t rof_steps_lds_bytes() {
 hipError_t launch_rof_steps(hipStream_t st, const Geometry& g, int nsteps,
                             const float4* u, const float4* p, const float4* q,
                             const float4* f, const float4* dt, float4 lambda,
-                            float4* u_out, float4* p_out, float4* q_out) {
+                            float hubereps, float4* u_out, float4* p_out,
+                            float4* q_out) {
   if (nsteps < 1 || nsteps > kRofK) return hipErrorInvalidValue;
-  PM_LAUNCH(k_rof_steps,
-            dim3((g.W + kRofTX - 1) / kRofTX, (g.H + kRofTY - 1) / kRofTY),
-            dim3(kRofRW, 4), 0, st, g.W, g.H, nsteps, u, p, q, f, dt, lambda,
-            u_out, p_out, q_out);
+  const dim3 grid((g.W + kRofTX - 1) / kRofTX, (g.H + kRofTY - 1) / kRofTY);
+  if (hubereps >= 0.0f)
+    PM_LAUNCH(k_rof_steps<true>, grid, dim3(kRofRW, 4), 0, st, g.W, g.H,
+              nsteps, hubereps, u, p, q, f, dt, lambda, u_out, p_out, q_out);
+  PM_LAUNCH(k_rof_steps<false>, grid, dim3(kRofRW, 4), 0, st, g.W, g.H,
+            nsteps, 0.0f, u, p, q, f, dt, lambda, u_out, p_out, q_out);
+}
+
+hipError_t launch_fill_state(hipStream_t st, const Geometry& g,
+                             const float4* states, float dispscale, int lr,
+                             float4* out) {
+  PM_LAUNCH(k_fill_state,
+            dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY), kBlock2D, 0,
+            st, g.W, g.H, states, dispscale, lr, out);
 }
 
 hipError_t launch_fill4(hipStream_t st, size_t n, float4 v, float4* out) {

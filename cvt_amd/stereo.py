@@ -16,7 +16,8 @@ import numpy as np
 
 from . import _native
 from ._native import (PM_PP_LRCHECK, PM_PP_LRCHECK_FILL, PM_PP_NONE,  # noqa
-                      PM_SMOOTH_HUBER, PM_SMOOTH_NONE, PmParams)
+                      PM_SMOOTH_DEPTHMAP, PM_SMOOTH_HUBER, PM_SMOOTH_NONE,
+                      PmParams)
 
 DISPARITY_PP_NONE = PM_PP_NONE
 DISPARITY_PP_LRCHECK = PM_PP_LRCHECK
@@ -285,20 +286,20 @@ class PMHuberStereo:
         self._ctx = None
         self._cfg = None
 
-    def _context(self, patchsize, depthmax):
-        cfg = (int(patchsize), float(depthmax))
+    def _context(self, patchsize, depthmax, smoothing=None):
+        smoothing = self.smoothing if smoothing is None else smoothing
+        cfg = (int(patchsize), float(depthmax), int(smoothing))
         if self._ctx is None or self._cfg != cfg:
             if self._ctx is not None:
                 self._ctx.close()
             self._ctx = Context(radius=cfg[0], disparity_max=cfg[1],
-                                smoothing=self.smoothing,
-                                device=self.device)
+                                smoothing=cfg[2], device=self.device)
             self._cfg = cfg
         return self._ctx
 
     def setup(self, left, right, patchsize, depthmax, initdisparity=None,
-              initnormalmap=None):
-        ctx = self._context(patchsize, depthmax)
+              initnormalmap=None, smoothing=None):
+        ctx = self._context(patchsize, depthmax, smoothing)
         ctx.set_images(left, right)
         if initdisparity is not None:
             ctx.setup_prior(initdisparity, initnormalmap)
@@ -323,6 +324,23 @@ class PMHuberStereo:
                          initnormalmap)
         ctx.iterate(int(iterations) | 1)
         d = ctx.disparity(pp, 1.0, 5.0)
+        if normalmap:
+            return d, ctx.normals(pp)
+        return d
+
+    def depthMap(self, left, right, patchsize, depthmax, iterations,
+                 dscale=-1.0, pp=DISPARITY_PP_LRCHECK, normalmap=False,
+                 initdisparity=None, initnormalmap=None):
+        """PMHuberStereo::depthMap (PMHuberStereo.cpp:147-347): its own
+        coupling (PM_SMOOTH_DEPTHMAP: Huber smoothing of filled fields,
+        theta schedule from 1e-3), exactly ``iterations`` iterations, the
+        result scaled by ``dscale`` (1 / depthmax when <= 0)."""
+        if dscale <= 0.0:
+            dscale = 1.0 / float(depthmax)
+        ctx = self.setup(left, right, patchsize, depthmax, initdisparity,
+                         initnormalmap, smoothing=PM_SMOOTH_DEPTHMAP)
+        ctx.iterate(int(iterations))
+        d = ctx.disparity(pp, 1.0, 5.0) * np.float32(dscale)
         if normalmap:
             return d, ctx.normals(pp)
         return d

@@ -92,10 +92,10 @@ class PMHuberStereo {
                     const Image* initdisparity = nullptr,
                     const Image* initnormalmap = nullptr);
 
-  // PMHuberStereo.h:50-51: the disparityMap result scaled by dscale
-  // (default 1 / depthmax).  The reference's depthMap runs its own variant
-  // of the pipeline (Huber instead of ROF smoothing, occlusion-aware fill,
-  // PMHuberStereo.cpp:147-347); this one keeps disparityMap's.
+  // PMHuberStereo.h:50-51, PMHuberStereo.cpp:147-347: depthMap's own
+  // coupling (PM_SMOOTH_DEPTHMAP: Huber smoothing of filled fields, theta
+  // from 1e-3), exactly `iterations` iterations, the disparity scaled by
+  // dscale (default 1 / depthmax).
   void depthMap(Image& dmap, const Image& left, const Image& right,
                 size_t patchsize, float depthmax, size_t iterations,
                 float dscale = -1.0f,
@@ -129,6 +129,8 @@ class PMHuberStereo {
 
   int device_;
   pm_smoothing smoothing_;
+  pm_smoothing mode_;      // coupling of the next context (depthMap switches)
+  pm_smoothing ctx_mode_ = PM_SMOOTH_NONE;
   pm_ctx* ctx_ = nullptr;
   int radius_ = -1;
   float dmax_ = -1.0f;

@@ -58,7 +58,14 @@ typedef enum pm_smoothing {
      * the distance to the smoothed field (pmhstereo.cl:846-877), and after
      * every iteration each view's (normal, depth) field is smoothed by 40
      * diffusion-tensor ROF steps (PDOptCL.cpp:1114-1168) */
-    PM_SMOOTH_HUBER = 1
+    PM_SMOOTH_HUBER = 1,
+    /* depthMap's pipeline (PMHuberStereo.cpp:147-347): theta starts at
+     * 1e-3 and follows ((i - 3) / (N - 4))^3 + 1e-3 after iteration
+     * i >= 3, the accept rule weighs theta * 10 times the distance, and
+     * each view's new field, filled by pmhstereo_fill_state, is smoothed
+     * by 40 diffusion-tensor Huber steps (PDOptCL.cpp:1170-1225, eps 1e-4,
+     * lambda theta * 15 + 2); theta_min is not used */
+    PM_SMOOTH_DEPTHMAP = 2
 } pm_smoothing;
 
 typedef struct pm_params {
@@ -177,7 +184,8 @@ int pm_normals(pm_ctx* ctx, int pp, float* normals);
 /* Wait for all work queued on the context. */
 int pm_sync(pm_ctx* ctx);
 
-/* PM_SMOOTH_HUBER: the smoothed (nx, ny, nz, disparity / disparity_max)
+/* PM_SMOOTH_HUBER, PM_SMOOTH_DEPTHMAP: the smoothed (nx, ny, nz,
+ * disparity / disparity_max)
  * field a view's accept rule currently pulls towards (the reference's
  * leftsmooth / rightsmooth, PMHuberStereo.cpp:439-440), W x H x 4. */
 int pm_get_smooth(pm_ctx* ctx, int view, float* out);

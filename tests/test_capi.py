@@ -67,7 +67,7 @@ def test_params_default_are_reference_constants(native):
     ("radius", 0), ("radius", 17), ("disparity_max", 0.0),
     ("disparity_max", float("nan")), ("disparity_max", float("inf")),
     ("color_weight", -1.0), ("theta_min", -0.5), ("theta_min", float("nan")),
-    ("smoothing", 2), ("schedule_iterations", -1), ("device", -1)])
+    ("smoothing", 3), ("schedule_iterations", -1), ("device", -1)])
 def test_create_rejects_bad_parameters(native, field, value):
     L = native.lib()
     p = native.PmParams()

@@ -218,3 +218,73 @@ def test_random_huber_configurations_bitexact(oracle, synth, W, H, C, r, dmax,
                "disp": ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)}
     for k, v in got.items():
         assert_same(v, ref[k], k)
+
+
+# ---- depthMap's coupling (PM_SMOOTH_DEPTHMAP) ------------------------------
+
+def depth_ctx(r, dmax, **kw):
+    return cvt_amd.Context(radius=r, disparity_max=dmax,
+                           smoothing=cvt_amd.PM_SMOOTH_DEPTHMAP, **kw)
+
+
+@pytest.mark.parametrize("which,r,dmax,n", [
+    ("rgba", 3, 20.0, 1),
+    ("rgba", 3, 20.0, 3),
+    ("rgba", 2, 20.0, 5),     # iterations 3 and 4 take the cubic schedule
+    ("gray", 4, 16.0, 3),
+])
+def test_depthmap_iterations_bitexact(oracle, pair, pair_gray, which, r, dmax,
+                                      n):
+    """depthMap's pipeline (PMHuberStereo.cpp:147-347): planes, the
+    Huber-smoothed filled fields and the disparity against
+    oracle.run_depthmap, bit for bit."""
+    L, R, _ = pair if which == "rgba" else pair_gray
+    print("depthmap pipeline", which, r, n, flush=True)
+    with depth_ctx(r, dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(n)
+        got = {"stateL": ctx.planes(0), "stateR": ctx.planes(1),
+               "smoothL": ctx.smooth(0), "smoothR": ctx.smooth(1),
+               "disp": ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)}
+    ref = oracle.run_depthmap(L, R, r, dmax, n, 1.0, 5.0)
+    for k, v in got.items():
+        assert_same(v, ref[k], k)
+
+
+def test_depthmap_full_size_bitexact(oracle, synth):
+    """depthMap's pipeline on the 720p bench pair, two iterations."""
+    L, R, _ = synth.make_pair(1280, 720, cells=24, dmax=128.0, channels=4,
+                              seed=0)
+    r, dmax, n = 4, 128.0, 2
+    print("depthmap full size", flush=True)
+    ref = oracle.run_depthmap(L, R, r, dmax, n, 1.0, 5.0)
+    with depth_ctx(r, dmax) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(n)
+        got = {"stateL": ctx.planes(0), "stateR": ctx.planes(1),
+               "smoothL": ctx.smooth(0), "smoothR": ctx.smooth(1),
+               "disp": ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)}
+    for k, v in got.items():
+        assert_same(v, ref[k], k)
+
+
+def test_depthmap_entry_scales_the_disparity(oracle, pair):
+    """cvt_amd.PMHuberStereo.depthMap: depthMap's pipeline for exactly the
+    given iteration count (no `| 1`), the disparity times dscale, and the
+    default dscale 1 / depthmax."""
+    L, R, _ = pair
+    ref = oracle.run_depthmap(L, R, 3, 20.0, 2, 1.0, 5.0)
+    pm = cvt_amd.PMHuberStereo()
+    try:
+        d = pm.depthMap(L, R, 3, 20.0, 2, dscale=0.5)
+        assert_same(d, ref["disp"] * np.float32(0.5), "dscale 0.5")
+        d = pm.depthMap(L, R, 3, 20.0, 2)
+        assert_same(d, ref["disp"] * np.float32(1.0 / 20.0), "default")
+        # the object's own pipeline is back for disparityMap
+        assert pm._cfg[2] == cvt_amd.PM_SMOOTH_DEPTHMAP
+        pm.disparityMap(L, R, 3, 20.0, 1)
+        assert pm._cfg[2] == cvt_amd.PM_SMOOTH_HUBER
+    finally:
+        pm.close()
