@@ -2,8 +2,10 @@
 // drives cvt::PMHuberStereo::disparityMap (cvt/vision/PMHuberStereo.h:47-48).
 //
 // usage: disparity_map W H C radius dmax iterations pp left.f32 right.f32
-//                      disp_out.f32 planes_out.f32 normals_out.f32 [plain]
-// ("plain": PatchMatch without the PM-Huber smoothing coupling)
+//                      disp_out.f32 planes_out.f32 normals_out.f32
+//                      [plain | depth]
+// ("plain": PatchMatch without the PM-Huber smoothing coupling; "depth":
+// depthMap with its default dscale instead of disparityMap)
 // Images are raw row-major float32 files; outputs are W x H (disparity)
 // and W x H x 4 (left plane field after the iterations, normal map).
 #include <cstdio>
@@ -49,11 +51,16 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "cannot read the input images\n");
       return 2;
     }
-    const bool plain = argc == 14 && std::string(argv[13]) == "plain";
-    cvt_amd::PMHuberStereo pm(0, plain ? PM_SMOOTH_NONE : PM_SMOOTH_HUBER);
-    pm.disparityMap(dmap, left, right, radius, dmax, iters,
-                    static_cast<cvt_amd::DisparityPostProcessing>(pp),
-                    &normals);
+    const std::string mode = argc == 14 ? argv[13] : "";
+    cvt_amd::PMHuberStereo pm(0, mode == "plain" ? PM_SMOOTH_NONE
+                                                 : PM_SMOOTH_HUBER);
+    const auto post = static_cast<cvt_amd::DisparityPostProcessing>(pp);
+    if (mode == "depth")
+      pm.depthMap(dmap, left, right, radius, dmax, iters, -1.0f, post,
+                  &normals);
+    else
+      pm.disparityMap(dmap, left, right, radius, dmax, iters, post,
+                      &normals);
     pm.planes(planes, 0);
     if (!write_raw(argv[10], dmap) || !write_raw(argv[11], planes) ||
         !write_raw(argv[12], normals)) {

@@ -37,7 +37,8 @@ def bits(a):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("pp,mode", [(1, "huber"), (2, "huber"), (1, "plain")])
+@pytest.mark.parametrize("pp,mode", [(1, "huber"), (2, "huber"), (1, "plain"),
+                                     (2, "depth")])
 def test_disparity_map_matches_oracle(tmp_path, oracle, synth, pp, mode):
     O = oracle
     W, H, C, R, dmax, iters = 72, 40, 4, 3, 14.0, 2
@@ -50,11 +51,13 @@ def test_disparity_map_matches_oracle(tmp_path, oracle, synth, pp, mode):
           flush=True)
     r = subprocess.run([exe, str(W), str(H), str(C), str(R), str(dmax),
                         str(iters), str(pp)] + paths +
-                       (["plain"] if mode == "plain" else []),
+                       ([mode] if mode in ("plain", "depth") else []),
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    # disparityMap makes the iteration count odd (PMHuberStereo.cpp:490)
-    assert "iterations run: 3" in r.stdout
+    # disparityMap makes the iteration count odd (PMHuberStereo.cpp:490),
+    # depthMap runs the count it is given
+    n = iters if mode == "depth" else iters | 1
+    assert "iterations run: %d" % n in r.stdout
     assert "mismatch:" in r.stdout
     d = np.fromfile(paths[2], np.float32).reshape(H, W)
     planes = np.fromfile(paths[3], np.float32).reshape(H, W, 4)
@@ -63,10 +66,14 @@ def test_disparity_map_matches_oracle(tmp_path, oracle, synth, pp, mode):
     # (PM-Huber, theta_min 1e-4)
     if mode == "huber":
         ref = O.run_huber(L, Rt, R, dmax, iters | 1, 1e-4, 1.0, 5.0)
+    elif mode == "depth":
+        ref = O.run_depthmap(L, Rt, R, dmax, iters, 1.0, 5.0)
     else:
         ref = O.run(L, Rt, R, dmax, iters | 1, 1.0, 5.0)
     np.testing.assert_array_equal(bits(planes), bits(ref["stateL"]))
     chk = O.consistency(ref["stateL"], ref["stateR"], 1.0, 5.0)
     want = O.to_disparity(chk) if pp == 1 else O.fill_depthmap(chk)
+    if mode == "depth":   # the default dscale, 1 / depthmax
+        want = want * (np.float32(1.0) / np.float32(dmax))
     np.testing.assert_array_equal(bits(d), bits(want))
     np.testing.assert_array_equal(bits(normals), bits(O.fill_normalmap(chk)))
