@@ -92,16 +92,23 @@ def test_huber_iterations_bitexact(oracle, pair, pair_gray, which, r, dmax,
     assert_same(d, ref["disp"], "disparity")
 
 
-def test_schedule_over_split_calls(pair):
-    """With schedule_iterations = N, iterating 2 + 1 gives what one call of
-    3 gives (theta = (i / N)^4 + theta_min after iteration i)."""
+@pytest.mark.parametrize("smoothing,n,split", [
+    (cvt_amd.PM_SMOOTH_HUBER, 3, (2, 1)),
+    (cvt_amd.PM_SMOOTH_DEPTHMAP, 6, (4, 2)),   # the cubic part after 3
+])
+def test_schedule_over_split_calls(pair, smoothing, n, split):
+    """With schedule_iterations = N, iterating in two calls gives what one
+    call of N gives (PM-Huber: theta = (i / N)^4 + theta_min after
+    iteration i; depthMap: ((i - 3) / (N - 4))^3 + 1e-3 from i = 3)."""
     L, R, _ = pair
     out = []
-    for split in ((3,), (2, 1)):
-        with huber_ctx(3, 20.0, schedule_iterations=3) as ctx:
+    for calls in ((n,), split):
+        with cvt_amd.Context(radius=3, disparity_max=20.0,
+                             smoothing=smoothing,
+                             schedule_iterations=n) as ctx:
             ctx.set_images(L, R)
             ctx.setup()
-            for k in split:
+            for k in calls:
                 ctx.iterate(k)
             out.append((ctx.planes(0), ctx.smooth(1)))
     assert_same(out[1][0], out[0][0], "planes")
