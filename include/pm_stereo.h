@@ -141,8 +141,8 @@ int pm_setup_prior(pm_ctx* ctx, const float* disparity, const float* normals);
  * call, so run the iterations as one call where possible (8 iterations of
  * a 720p pair: 52.0 ms as one call, 56.9 ms as eight calls of one).  With
  * PM_SMOOTH_NONE the call can be recorded by stream capture on the
- * context's stream; with PM_SMOOTH_HUBER a capturing stream gets
- * PM_ERR_ARG before anything is queued. */
+ * context's stream; with PM_SMOOTH_HUBER or PM_SMOOTH_DEPTHMAP a
+ * capturing stream gets PM_ERR_ARG before anything is queued. */
 int pm_iterate(pm_ctx* ctx, int n);
 int pm_iteration(const pm_ctx* ctx);
 
