@@ -5,16 +5,20 @@ Workload (one step = one stereo pair, inputs resident in HBM): a seeded
 synthetic 1280x720 RGBA float32 pair with 24 slanted planar cells and
 disparities in 0-128 px; precompute (bilateral weights, gradients), random
 plane init of both views, 8 PatchMatch iterations (left then right view,
-one pm_iterate call) with a 9x9 patch, LR check at 1 px (angle test off)
-and disparity conversion, ordered on the engine's stream (the engine runs
-each view's propagate as two half launches on a second stream of its own
-and joins it back).
+one pm_iterate call) with a 9x9 patch and the gamma-10 truncated
+colour+gradient cost (patch_eval_color_grad_weighted, support-weight gain
+255 / 10 = 25.5 on [0, 1] data), LR check at 1 px (angle test off) and
+disparity conversion, ordered on the engine's stream (the engine runs each
+view's propagate as two half launches on a second stream of its own and
+joins it back).  --cost census runs the reference's default census cost
+instead.
 
 Prints one JSON line with the throughput, the roofline of the dominant
 kernel (k_propagate: FP32 VALU bound, algorithmic FLOPs from an exact tap
 count, per view propagate) and the CPU baseline (the oracle on the host
-cores, timed on a bounded sample of the same workload).  --pipeline huber
-runs the PM-Huber pipeline instead and quotes the ROF kernel it adds.
+cores: one full run of the same pair, and the 320x240 gray configuration
+as one warm-up plus the median of five full runs).  --pipeline huber runs
+the PM-Huber pipeline instead and quotes the ROF kernel it adds.
 """
 import argparse
 import json
@@ -29,13 +33,24 @@ METRIC = ("720p stereo pairs/s at matched bad-1.0px; learned-init vs plain "
           "PatchMatch")
 W, H, CELLS, DMAX, RADIUS, ITERS, SEED = 1280, 720, 24, 128.0, 4, 8, 0
 MAXDISP, MAXDEG = 1.0, 180.0
-SAMPLE_ITERS = 3     # iterations the CPU baseline times (of ITERS)
-# algorithmic FP32 work per valid tap (FMA = 2 FLOP):
-#  cost tap  = transform 4 + colour lerp 10 + gradient lerp 13 + colour term
-#              13 + sign term 6 + accumulate/exit 4                 = 50
-#  KLT tap   = 3 lerped colour/gradient samples + 3 cost terms + the
-#              outer-product accumulate                             = 150
-FLOP_PER_COST_TAP = 50
+# the small CPU configuration: 320x240 gray, 8 cells, disparity 0-32, 11x11
+# patch, 3 iterations (census cost, the reference's)
+C1 = dict(W=320, H=240, cells=8, dmax=32.0, radius=5, iters=3, seed=1)
+C1_RUNS = 5
+# candidate costs: the gamma-10 colour+gradient cost (255 / 10 on [0, 1]
+# data) and the reference's census cost (weight CENSUS_COLOR_WEIGHT 12)
+COSTS = {"color_grad": (1, 25.5), "census": (0, 16.0)}
+# algorithmic FP32 work per valid tap (FMA = 2 FLOP, abs and min count 1):
+#  census tap     = transform 4 + colour lerp 10 + gradient lerp 13 +
+#                   colour term 13 + sign term 6 + accumulate/exit 4   = 50
+#  colour+grad tap = transform 4 + colour lerp 10 + gradient lerp 13 +
+#                   colour term (3 sub, 3 abs, 3 min, 2 add) 11 +
+#                   gradient term (4 sub, 4 abs, 4 min, 3 add) 15 +
+#                   the alpha blend (mul + fma) 3 + accumulate (fma +
+#                   add) 3                                             = 59
+#  KLT tap        = 3 lerped colour/gradient samples + 3 cost terms +
+#                   the outer-product accumulate                      = 150
+FLOP_PER_COST_TAP = {"census": 50, "color_grad": 59}
 FLOP_PER_KLT_TAP = 150
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0
@@ -95,36 +110,53 @@ def host_cpu():
     return {"cpu_model": model, "nproc": nproc, "threads": O.num_threads()}
 
 
-def cpu_baseline(L, R, gt):
-    """The oracle (C, OpenMP on all host cores) on a bounded sample: the
-    full setup, SAMPLE_ITERS of the eight iterations and the
-    post-processing of the same pair; per-pair time = setup + 8 x the mean
-    iteration + post."""
+def cpu_baseline(L, R, gt, cost, gpu_disp):
+    """The oracle (C, OpenMP on all host cores): one full run of the 720p
+    pair (setup, all 8 iterations, LR check and disparity, the GPU step's
+    sequence), and the 320x240 gray configuration as one warm-up plus the
+    median of C1_RUNS full runs.  Also reports how the oracle's disparity
+    compares with the GPU's (0.5 px bar, bit equality)."""
     import numpy as np
+
+    from cvt_amd import disparity, synth
     from oracle import oracle as O
+    kind, kw = COSTS[cost]
     t0 = time.perf_counter()
-    tl, tr = O.prepare(L, RADIUS), O.prepare(R, RADIUS)
-    sl = O.init(tl, tr, RADIUS, DMAX, 1)
-    sr = O.init(tr, tl, RADIUS, DMAX, 0)
+    ref = O.run(L, R, RADIUS, DMAX, ITERS, maxdisp=MAXDISP, maxdeg=MAXDEG,
+                cost_kind=kind, cost_weight=kw)
     t1 = time.perf_counter()
-    vb2 = O.viewbuf_empty(W, H)
-    for it in range(SAMPLE_ITERS):
-        sl, vb1 = O.propagate(tl, tr, RADIUS, DMAX, 1, it, sl, vb2, sr)
-        sr, vb2 = O.propagate(tr, tl, RADIUS, DMAX, 0, it, sr, vb1, sl)
-    t2 = time.perf_counter()
-    d = O.to_disparity(O.consistency(sl, sr, MAXDISP, MAXDEG))
-    t3 = time.perf_counter()
-    per_iter = (t2 - t1) / SAMPLE_ITERS
-    per_pair = (t1 - t0) + ITERS * per_iter + (t3 - t2)
-    del d, np
+    per_pair = t1 - t0
+    d = ref["disp"]
+    within = float((np.abs(d - gpu_disp) <= 0.5).mean())
+    identical = bool(np.array_equal(d.view(np.uint32),
+                                    gpu_disp.view(np.uint32)))
+    cL, cR, cgt = synth.make_pair(C1["W"], C1["H"], cells=C1["cells"],
+                                  dmax=C1["dmax"], channels=1,
+                                  seed=C1["seed"])
+    times = []
+    for k in range(C1_RUNS + 1):
+        s0 = time.perf_counter()
+        c1 = O.run(cL, cR, C1["radius"], C1["dmax"], C1["iters"],
+                   maxdisp=MAXDISP, maxdeg=MAXDEG)
+        s1 = time.perf_counter()
+        if k > 0:                       # the first run is the warm-up
+            times.append(s1 - s0)
+    c1_s = float(np.median(times))
     return {"value": 1.0 / per_pair, "unit": "pairs/s",
             "cores": O.num_threads(), "kind": "port", "host": host_cpu(),
-            "sample": "1280x720 pair: setup + %d of %d iterations + LR "
-                      "check timed (%.1f s); per pair = setup %.2f s + %d x "
-                      "iteration %.2f s + post %.3f s" %
-                      (SAMPLE_ITERS, ITERS, t3 - t0, t1 - t0, ITERS, per_iter,
-                       t3 - t2),
-            "ms_per_pair": per_pair * 1e3}
+            "sample": "the same 1280x720 pair, one full run (setup + %d "
+                      "iterations + LR check: %.1f s)" % (ITERS, per_pair),
+            "ms_per_pair": per_pair * 1e3,
+            "bad_1px": disparity.bad_pixel_rate(d, gt, 1.0),
+            "gpu_disp_within_0.5px": within,
+            "gpu_disp_bit_identical": identical,
+            "c1": {"workload": "320x240 gray f32 pair, 8 cells, disparity "
+                               "0-32 px, 11x11 patch, 3 iterations, census "
+                               "cost, LR check 1 px",
+                   "pairs_per_s": 1.0 / c1_s, "ms_per_pair": c1_s * 1e3,
+                   "runs": "1 warm-up + median of %d" % C1_RUNS,
+                   "bad_1px": disparity.bad_pixel_rate(c1["disp"], cgt,
+                                                       1.0)}}
 
 
 # algorithmic HBM bytes per pixel of one k_rof_steps launch (two ROF
@@ -208,6 +240,10 @@ def main():
                          "PM-Huber pipeline (theta schedule plus "
                          "diffusion-tensor ROF smoothing after every "
                          "iteration)")
+    ap.add_argument("--cost", choices=sorted(COSTS), default="color_grad",
+                    help="candidate cost: the gamma-10 truncated "
+                         "colour+gradient cost (default) or the reference's "
+                         "census cost")
     ap.add_argument("--io-steps", type=int, default=2,
                     help="pairs run from and to host memory after the timed "
                          "region (PCIe-inclusive rate)")
@@ -260,10 +296,12 @@ def main():
     dD = torch.empty((H, W), dtype=torch.float32, device=dev)
     stream = torch.cuda.Stream(device=dev)
     huber = args.pipeline == "huber"
+    cost_kind, cost_weight = COSTS[args.cost]
     ctx = cvt_amd.Context(
         radius=RADIUS, disparity_max=DMAX, stream=stream.cuda_stream,
         smoothing=cvt_amd.PM_SMOOTH_HUBER if huber else cvt_amd.PM_SMOOTH_NONE,
-        schedule_iterations=ITERS)
+        schedule_iterations=ITERS, cost_kind=cost_kind,
+        cost_weight=cost_weight)
 
     def step(events=None):
         ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
@@ -317,8 +355,8 @@ def main():
     ctx.close()
 
     propagates = 2 * ITERS
-    flop = (FLOP_PER_COST_TAP * cost_taps + FLOP_PER_KLT_TAP * klt_taps) \
-        / propagates
+    flop = (FLOP_PER_COST_TAP[args.cost] * cost_taps +
+            FLOP_PER_KLT_TAP * klt_taps) / propagates
     achieved = flop / (prop_ms * 1e-3) / 1e12
     elapsed = t1 - t0
     value = args.steps / elapsed
@@ -338,8 +376,12 @@ def main():
         "data": "synthetic (seeded Voronoi slanted planes, Perlin texture)",
         "config": {"workload": "plain PatchMatch, 1280x720 RGBA f32 pair, 24 "
                                "planar cells, disparity 0-128 px, 9x9 patch, "
-                               "8 iterations, LR check 1 px",
-                   "variant": "plain", "width": W, "height": H,
+                               "%s, 8 iterations, LR check 1 px" %
+                               ("gamma-10 truncated colour+gradient cost "
+                                "(weight 25.5)" if args.cost == "color_grad"
+                                else "census cost (weight 12)"),
+                   "variant": "plain", "cost": args.cost,
+                   "cost_weight": cost_weight, "width": W, "height": H,
                    "radius": RADIUS, "iterations": ITERS,
                    "disparity_max": DMAX, "seed": SEED},
         "quality": {"bad_1px": bad1},
@@ -382,7 +424,8 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) on a bounded sample")
         result["cpu_baseline"] = (cpu_baseline_huber(L, R) if huber
-                                  else cpu_baseline(L, R, gt))
+                                  else cpu_baseline(L, R, gt, args.cost,
+                                                    disp))
     print(json.dumps(result), flush=True)
     return 0
 

@@ -13,6 +13,7 @@ SO_PATH = os.path.join(HERE, "libcvt_amd_pm.so")
 PM_OK, PM_ERR_ARG, PM_ERR_STATE, PM_ERR_HIP, PM_ERR_NOMEM, PM_ERR_IO = range(6)
 PM_PP_NONE, PM_PP_LRCHECK, PM_PP_LRCHECK_FILL = range(3)
 PM_SMOOTH_NONE, PM_SMOOTH_HUBER, PM_SMOOTH_DEPTHMAP = range(3)
+PM_COST_CENSUS, PM_COST_COLOR_GRAD = range(2)
 
 _p = ctypes.c_void_p
 _i = ctypes.c_int
@@ -24,6 +25,8 @@ class PmParams(ctypes.Structure):
     _fields_ = [("radius", ctypes.c_int),
                 ("disparity_max", ctypes.c_float),
                 ("color_weight", ctypes.c_float),
+                ("cost_kind", ctypes.c_int),
+                ("cost_weight", ctypes.c_float),
                 ("smoothing", ctypes.c_int),
                 ("theta_min", ctypes.c_float),
                 ("schedule_iterations", ctypes.c_int),
@@ -61,6 +64,18 @@ SIGNATURES = {
     "pm_get_tensor": (_i, [_p, _i, _p]),
     "pm_rof": (_i, [_p, _i, _p, _p, _i, _p]),
     "pm_rof_device": (_i, [_p, _i, _p, _p, _i, _p]),
+    # include/pm_net.h
+    "pm_net_create": (_i, [_i, _p, _u64, ctypes.POINTER(_p)]),
+    "pm_net_destroy": (None, [_p]),
+    "pm_net_last_error": (ctypes.c_char_p, [_p]),
+    "pm_net_num_params": (ctypes.c_size_t, [_p]),
+    "pm_net_get_params": (_i, [_p, _p, ctypes.c_size_t]),
+    "pm_net_set_params": (_i, [_p, _p, ctypes.c_size_t]),
+    "pm_net_get_grads": (_i, [_p, _p, ctypes.c_size_t]),
+    "pm_net_forward": (_i, [_p, _p, _i, _i, _i, _p]),
+    "pm_net_train_step": (_i, [_p, _p, _p, _i, _i, _i, _f, _f, _f,
+                               ctypes.POINTER(_f)]),
+    "pm_setup_proposal": (_i, [_p, _p]),
     # include/pm_io.h
     "pm_io_last_error": (ctypes.c_char_p, []),
     "pm_pfm_info": (_i, [ctypes.c_char_p, _p, _p, _p]),

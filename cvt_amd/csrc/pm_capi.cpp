@@ -18,6 +18,7 @@
 #include <string>
 
 #include "pm_engine.h"
+#include "pm_net_internal.h"
 #include "pm_stereo.h"
 
 namespace {
@@ -156,6 +157,10 @@ int check_params(pm_ctx* c, const pm_params* p) {
     return fail(c, PM_ERR_ARG, "disparity_max must be finite and > 0");
   if (!(p->color_weight >= 0.0f) || !std::isfinite(p->color_weight))
     return fail(c, PM_ERR_ARG, "color_weight must be finite and >= 0");
+  if (p->cost_kind != PM_COST_CENSUS && p->cost_kind != PM_COST_COLOR_GRAD)
+    return fail(c, PM_ERR_ARG, "unknown cost_kind %d", p->cost_kind);
+  if (!(p->cost_weight >= 0.0f) || !std::isfinite(p->cost_weight))
+    return fail(c, PM_ERR_ARG, "cost_weight must be finite and >= 0");
   if (p->smoothing != PM_SMOOTH_NONE && p->smoothing != PM_SMOOTH_HUBER &&
       p->smoothing != PM_SMOOTH_DEPTHMAP)
     return fail(c, PM_ERR_ARG, "unknown smoothing %d", p->smoothing);
@@ -213,6 +218,10 @@ int alloc_pair(pm_ctx* c, int W, int H, int channels) {
   c->geo.TS = pm::tex_stride(W);
   c->geo.npix = n;
   c->geo.color_weight = c->prm.color_weight;
+  c->geo.cost_kind = c->prm.cost_kind == PM_COST_COLOR_GRAD
+                         ? pm::kCostColorGrad
+                         : pm::kCostCensus;
+  c->geo.cost_weight = c->prm.cost_weight;
   c->channels = channels;
   return PM_OK;
 }
@@ -515,14 +524,22 @@ int smooth_after(pm_ctx* c, int view, const float4* planes, float theta,
 // theta after iteration i (0-based) of a schedule of c->schedule_n:
 // disparityMapAll's (i / N)^4 + theta_min (PMHuberStereo.cpp:575-576), or
 // depthMap's ((i - 3) / (N - 4))^3 + 1e-3 from i = 3 on, unchanged before
-// (PMHuberStereo.cpp:315-318)
+// (PMHuberStereo.cpp:315-318).  The reference uses theta only for the
+// iterations of its schedule (i < N - 1 for the value computed after
+// iteration i), where the fraction lies in [0, 1).  Split pm_iterate calls
+// can run past the schedule (i >= N - 1, or depthMap's N <= 4, whose
+// 0 / 0 would give NaN): theta then holds at the schedule's end, the
+// fraction clamped to 1.
 float theta_after(const pm_ctx* c, int i, float theta) {
   if (depthmap_mode(c)) {
     if (i < 3) return theta;
-    const float t = ((float)i - 3.0f) / ((float)c->schedule_n - 4.0f);
+    const float den = (float)c->schedule_n - 4.0f;
+    float t = den > 0.0f ? ((float)i - 3.0f) / den : 1.0f;
+    t = std::min(t, 1.0f);
     return t * t * t + kDepthTheta0;
   }
-  return std::pow((float)i / (float)c->schedule_n, 4.0f) + c->prm.theta_min;
+  const float t = std::min((float)i / (float)c->schedule_n, 1.0f);
+  return std::pow(t, 4.0f) + c->prm.theta_min;
 }
 
 // pmhstereo_propagate_view for the left then the right view, each followed
@@ -701,6 +718,8 @@ void pm_params_default(pm_params* p) {
   p->radius = 5;
   p->disparity_max = 64.0f;
   p->color_weight = 12.0f;
+  p->cost_kind = PM_COST_CENSUS;
+  p->cost_weight = 16.0f;
   p->smoothing = PM_SMOOTH_NONE;
   p->theta_min = 1e-4f;
   p->schedule_iterations = 0;
@@ -811,6 +830,43 @@ int pm_setup(pm_ctx* c) {
   PM_HIP(c, pm::launch_init(c->stream, c->geo, view_bufs(c, 1),
                             view_bufs(c, 0), c->prm.disparity_max, 0,
                             state_ptr(c, 1, 0)));
+  PM_HIP(c, pm::launch_viewbuf_clear(
+                c->stream, c->geo,
+                static_cast<unsigned long long*>(c->vbuf[1].ptr)));
+  if ((rc = huber_setup(c))) return rc;
+  c->cur = 0;
+  c->iteration = 0;
+  c->ready = true;
+  c->last_pp = -1;
+  return PM_OK;
+}
+
+int pm_setup_proposal(pm_ctx* c, pm_net* net) {
+  if (!c) return PM_ERR_ARG;
+  if (!net) return fail(c, PM_ERR_ARG, "net is NULL");
+  int rc;
+  if ((rc = require_images(c)) || (rc = use_device(c))) return rc;
+  if (pm::net::device_of(net) != c->prm.device)
+    return fail(c, PM_ERR_ARG, "the net is on device %d, the context on %d",
+                pm::net::device_of(net), c->prm.device);
+  if ((rc = prepare_views(c))) return rc;
+  const float* img[2] = {static_cast<const float*>(c->img[0].ptr),
+                         static_cast<const float*>(c->img[1].ptr)};
+  const int W = c->geo.W, H = c->geo.H, C = c->channels;
+  for (int view = 0; view < 2; ++view) {
+    const float* prop = nullptr;
+    int Wp = 0, os = 0;
+    // the right view's proposal: the network on the mirrored pair
+    if (pm::net::proposal(net, c->stream, img[0], img[1], W, H, C, W * C,
+                          view, &prop, &Wp, &os))
+      return fail(c, PM_ERR_HIP, "proposal network: %s",
+                  pm::net::error_of(net));
+    PM_HIP(c, pm::launch_init_proposal(
+                  c->stream, c->geo, view_bufs(c, view),
+                  view_bufs(c, 1 - view), c->prm.disparity_max,
+                  view == 0 ? 1 : 0, prop, os, Wp, PM_NET_DISP_SCALE,
+                  state_ptr(c, view, 0)));
+  }
   PM_HIP(c, pm::launch_viewbuf_clear(
                 c->stream, c->geo,
                 static_cast<unsigned long long*>(c->vbuf[1].ptr)));

@@ -21,7 +21,6 @@ constexpr float kNormalRefineMul = 0.05f;   // NORMALREFINEMUL, pmhstereo.cl:31
 constexpr int kNumRndSample = 4;            // NUMRNDSAMPLE, pmhstereo.cl:34
 constexpr float kRndSampleRadius = 5.0f;    // NUMRNDSAMPLERADIUS, :35
 constexpr int kViewSamples = 2;             // VIEWSAMPLES, pmhstereo.cl:46
-constexpr int kPropDraws = 14;              // draws per pixel, pmhstereo.cl:942
 constexpr int kKltPasses = 6;               // pmhstereo.cl:1036-1038
 constexpr uint32_t kSlotEmpty = 0xFFFFFFFFu;
 
@@ -143,6 +142,17 @@ __device__ __forceinline__ uint64_t mulmod_m(uint64_t a, uint64_t b) {
 struct Mwc {
   uint32_t x, c;
 };
+
+// Stream index of the first draw of a propagate at pixel p, iteration
+// iter.  The reference calls RNG_init(&rng, p + iter, <numdraws text>)
+// (pmhstereo.cl:942); the macro expands to Skip((linpos) * numdraws)
+// (RNG.cl:54-57) with the argument text unparenthesised, which with
+// PROPSIZE 0, NUMRNDTRIES 2 and NUMRNDSAMPLE 4 evaluates to
+// (p + iter) * 2 * 3 + 2 * 4.  A propagate consumes 14 draws, so the
+// windows of neighbouring pixels overlap.
+__device__ __forceinline__ uint64_t prop_draw_start(uint64_t p, int iter) {
+  return (p + (uint64_t)iter) * 6u + 8u;
+}
 
 // MWC64X_NextFloat (RNG.cl:30-35) + MWC64X_Step (MWC64X.cl:109-120)
 __device__ __forceinline__ float mwc_next(Mwc* s) {

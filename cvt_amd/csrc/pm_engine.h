@@ -36,6 +36,13 @@ __host__ __device__ inline int tex_stride(int W) {
 constexpr int kBlockCols = 64;
 constexpr int kBlockRows = 4;
 
+// candidate cost of the plane-evaluating kernels: what the reference's
+// `#define patch_eval` (pmhstereo.cl:753) selects
+enum CostKind {
+  kCostCensus = 0,     // patch_eval_census, pmhstereo.cl:661-736
+  kCostColorGrad = 1   // patch_eval_color_grad_weighted, pmhstereo.cl:510-565
+};
+
 struct Geometry {
   int W = 0, H = 0, R = 0;
   int TS = 0;  // texel row stride in records, tex_stride(W)
@@ -43,7 +50,12 @@ struct Geometry {
   // a launch covers the block rows [by0, by0 + nby) of the image
   // (nby = 0: all of them)
   int by0 = 0, nby = 0;
+  // CENSUS_COLOR_WEIGHT: bilateral weight sum, census cost and KLT step
   float color_weight = 12.0f;
+  int cost_kind = kCostCensus;
+  // support-weight factor of the colour+gradient cost (16 in the
+  // reference, pmhstereo.cl:531)
+  float cost_weight = 16.0f;
 };
 
 hipError_t launch_prepare(hipStream_t st, const Geometry& g, const float* img,
@@ -59,6 +71,14 @@ hipError_t launch_init_prior(hipStream_t st, const Geometry& g, ViewBufs self,
                              ViewBufs other, float dmax, int lr,
                              const float* disp, const float4* normals,
                              float4* states);
+
+// init of a view from a plane proposal (Hp x Wp x os fp32 per pixel:
+// disparity / dscale, normal; the right view's from the mirrored pass)
+// with a per-pixel fallback to the random plane of launch_init
+hipError_t launch_init_proposal(hipStream_t st, const Geometry& g,
+                                ViewBufs self, ViewBufs other, float dmax,
+                                int lr, const float* prop, int os, int Wp,
+                                float dscale, float4* states);
 
 // clear the view-scatter slots of image rows [row0, row0 + rows)
 hipError_t launch_viewbuf_clear_rows(hipStream_t st, const Geometry& g,

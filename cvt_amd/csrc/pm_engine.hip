@@ -78,9 +78,12 @@ __host__ __forceinline__ dim3 grid_for(const Geometry& g) {
   return dim3((unsigned)((g.W + kTX - 1) / kTX), (unsigned)nby);
 }
 
-// LDS bytes of the block kernels: the own-view tile
-__host__ __forceinline__ size_t block_lds_bytes(int R) {
-  return (size_t)(kTX + 2 * R) * (kTY + 2 * R) * sizeof(float4);
+// LDS bytes of the block kernels: the own-view tile (colour and sign
+// bits), followed for the colour+gradient cost by a tile of the own view's
+// gradients
+__host__ __forceinline__ size_t block_lds_bytes(int R, int kind) {
+  const size_t tile = (size_t)(kTX + 2 * R) * (kTY + 2 * R) * sizeof(float4);
+  return kind == kCostColorGrad ? 2 * tile : tile;
 }
 
 // sign bits of a gradient as one byte per component: byte k is 0xff when
@@ -117,14 +120,20 @@ struct Lane {
   const float4* trow0;  // LDS tile at this lane: tap (dx, dy) is
                         // trow0[(dy + R) * TW + dx + R]
   int TW;
+  int GT;               // offset of the gradient tile (records), when
+                        // staged: tap (dx, dy)'s gradient is at
+                        // trow0[GT + (dy + R) * TW + dx + R]
+  float kw;             // colour+gradient cost's support-weight factor
   int TS;               // texel row stride (records, with the edge pads)
   size_t GP;            // offset of the gradient plane (records)
   const float4* __restrict__ tex2;  // other view's colour record (0, 0)
 };
 
 // Block prologue shared by the cost-evaluating kernels: stages the own-view
-// tile (r, g, b, gradient sign bits) and returns the lane context.  Must be
-// reached by every thread of the block.
+// tile (r, g, b, gradient sign bits), and for kGrad the tile of its
+// gradients after it, and returns the lane context.  Must be reached by
+// every thread of the block.
+template <bool kGrad>
 __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
                             const ViewBufs& other, float4* smem) {
   int bx, by;
@@ -134,15 +143,19 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
   const int TW = kTX + 2 * R, TH = kTY + 2 * R;
   const int x0 = bx * kTX, y0 = by * kTY;
   const int tid = threadIdx.y * kTX + threadIdx.x;
+  const size_t gplane = (size_t)g.TS * H;
   for (int i = tid; i < TW * TH; i += kTX * kTY) {
     const int gx = x0 - R + i % TW, gy = y0 - R + i / TW;
-    float4 v = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 v = mk4(0.0f, 0.0f, 0.0f, 0.0f), gv = v;
     if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-      const float4 a = self.tex[(size_t)gy * g.TS + kTexPadL + gx];
+      const size_t rec = (size_t)gy * g.TS + kTexPadL + gx;
+      const float4 a = self.tex[rec];
       v = mk4(a.x, a.y, a.z,
               __uint_as_float(expand_signs(self.sgn[(size_t)gy * W + gx])));
+      if (kGrad) gv = self.tex[gplane + rec];
     }
     smem[i] = v;
+    if (kGrad) smem[TW * TH + i] = gv;
   }
   __syncthreads();
   // the row index is uniform per wavefront: keep it scalar
@@ -160,6 +173,8 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
   L.yf = (float)L.y;
   L.p = L.valid ? (size_t)L.y * W + L.x : 0;
   L.TW = TW;
+  L.GT = TW * TH;
+  L.kw = g.cost_weight;
   L.TS = g.TS;
   L.GP = (size_t)g.TS * g.H;
   L.trow0 = smem + ly * TW + threadIdx.x;
@@ -292,6 +307,72 @@ __device__ float cost_eval(const Lane& L, float4 s, float mincost,
   }
   if (live) res = (wsum <= 1.0f) ? 1.1f : census / wsum;
   return res;
+}
+
+// patch_eval_color_grad_weighted (pmhstereo.cl:510-565), one plane per lane
+// (lanes with active == false idle), patch rows wave-uniform:
+// w = exp(-|dot(I1(p) - I1(q), gray)| * kw), C = 0.05 sum_3 min(|I1(q) -
+// I2(x')|, 0.05) + 0.95 sum_4 min(|G1(q) - G2(x')|, 0.05), result
+// sum(w C) / sum(w).  No early exit and no invalid-plane test: a tap counts
+// when its own pixel is in the image and x' lies in [0, W) (a non-finite x'
+// fails the test; the reference samples at it, which OpenCL leaves
+// undefined).  No qualifying tap gives 0 / 0 = NaN, which the accept rule
+// never takes.  Needs the gradient tile (block_setup<true>).
+// kCount: count the taps evaluated.
+template <bool kCount>
+__device__ float cost_eval_cg(const Lane& L, float4 s, bool active,
+                              uint32_t* ntaps) {
+  const int R = L.R, W = L.W;
+  const bool live = active && L.valid;
+  const int dx0 = max(-R, -L.x), dx1 = min(R, W - 1 - L.x);
+  float ret = 0.0f, wsum = 0.0f;
+  for (int dy = -R; dy <= R; ++dy) {
+    if (!__any(live)) break;
+    const int qy = L.y + dy;  // uniform
+    if (qy < 0 || qy >= L.H) continue;
+    if (!live) continue;
+    const float rowb = s.y * (float)qy;  // transform(), b y per row
+    const float4* __restrict__ srow = L.tex2 + (size_t)qy * L.TS;
+    const float4* trow = L.trow0 + (dy + R) * L.TW + R;
+    const float4* grow = trow + L.GT;
+    float qx = L.xf + (float)dx0;
+    for (int dx = dx0; dx <= dx1; ++dx, qx += 1.0f) {
+      const float xq = fmaf(s.x, qx, rowb) + s.z;
+      if (!(xq >= 0.0f && xq < L.Wf)) continue;
+      const float4 t1 = trow[dx];
+      const float4 g1 = grow[dx];
+      // dot(valcenter - val1, gray): the negated difference gives the
+      // negated dot bit for bit, so |.| is the same
+      const float w = support_weight(t1.x - L.tc.x, t1.y - L.tc.y,
+                                     t1.z - L.tc.z, L.kw);
+      const float fq = floorf(xq);
+      const float4* __restrict__ tq = srow + (int)fq;
+      const Sample s2 =
+          lerp_texels(texel_at(tq, L.GP), texel_at(tq + 1, L.GP), xq - fq);
+      const float sc = (fminf(fabsf(t1.x - s2.r), 0.05f) +
+                        fminf(fabsf(t1.y - s2.g), 0.05f)) +
+                       fminf(fabsf(t1.z - s2.b), 0.05f);
+      const float sg = ((fminf(fabsf(g1.x - s2.grad.x), 0.05f) +
+                         fminf(fabsf(g1.y - s2.grad.y), 0.05f)) +
+                        fminf(fabsf(g1.z - s2.grad.z), 0.05f)) +
+                       fminf(fabsf(g1.w - s2.grad.w), 0.05f);
+      // alpha dot(.) + (1 - alpha) dot(.), the left product fused
+      const float C = fmaf(0.05f, sc, (1.0f - 0.05f) * sg);
+      wsum += w;
+      ret = fmaf(w, C, ret);
+      if (kCount) ++*ntaps;
+    }
+  }
+  return ret / wsum;
+}
+
+// the candidate cost the kernels' patch_eval selects
+template <int kKind, bool kCount>
+__device__ __forceinline__ float cost_any(const Lane& L, float4 s,
+                                          float mincost, bool active,
+                                          uint32_t* ntaps) {
+  if (kKind == kCostColorGrad) return cost_eval_cg<kCount>(L, s, active, ntaps);
+  return cost_eval<kCount>(L, s, mincost, active, ntaps);
 }
 
 // one census term of patch_census_klt2 (pmhstereo.cl:360-392); v0..v2 is
@@ -473,18 +554,19 @@ k_prepare(const float* __restrict__ img, int W, int H, int C, int stride,
 }
 
 // pmhstereo_init (pmhstereo.cl:755-773): RNG_init(p, 3), three draws
+template <int kKind>
 __global__ void __launch_bounds__(256)
 k_init(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
        float4* __restrict__ states) {
   extern __shared__ float4 smem[];
-  const Lane L = block_setup(g, self, other, smem);
+  const Lane L = block_setup<kKind == kCostColorGrad>(g, self, other, smem);
   float4 s = mk4(0.0f, 0.0f, 0.0f, 0.0f);
   if (L.valid) {
     Mwc rng = mwc_jump((uint64_t)L.p * 3u);
     const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
     s = state_init(u0, u1, u2, L.xf, L.yf, lr, dmax);
   }
-  const float w = cost_eval<false>(L, s, 1e10f, true, nullptr);
+  const float w = cost_any<kKind, false>(L, s, 1e10f, true, nullptr);
   if (L.valid) {
     s.w = w;
     states[L.p] = s;
@@ -510,12 +592,13 @@ k_disparity_to_normal(int W, int H, const float* __restrict__ disp,
 }
 
 // pmhstereo_init_disparity_normal (pmhstereo.cl:802-828)
+template <int kKind>
 __global__ void __launch_bounds__(256)
 k_init_prior(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
              const float* __restrict__ disp,
              const float4* __restrict__ normals, float4* __restrict__ states) {
   extern __shared__ float4 smem[];
-  const Lane L = block_setup(g, self, other, smem);
+  const Lane L = block_setup<kKind == kCostColorGrad>(g, self, other, smem);
   float4 s = mk4(0.0f, 0.0f, 0.0f, 0.0f);
   if (L.valid) {
     Mwc rng = mwc_jump((uint64_t)L.p * 3u);
@@ -530,10 +613,55 @@ k_init_prior(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
       s = state_init(u0, u1, u2, L.xf, L.yf, lr, dmax);
     }
   }
-  const float w = cost_eval<false>(L, s, 1e10f, true, nullptr);
+  const float w = cost_any<kKind, false>(L, s, 1e10f, true, nullptr);
   if (L.valid) {
     s.w = w;
     states[L.p] = s;
+  }
+}
+
+// Init of a view from a plane proposal, with a per-pixel fallback: the
+// proposed plane (disparity d = o.x * dscale and normal (o.y, o.z, o.w) at
+// the pixel, from_disparity_normal as pmhstereo_init_disparity_normal
+// builds it, pmhstereo.cl:802-828) competes with the random plane of
+// pmhstereo_init (draws 3p..3p+2, pmhstereo.cl:755-773), and the random
+// plane takes the pixel only when its cost is lower (a NaN cost never
+// wins).  The right view's proposal (lr = 0) comes from the network's pass
+// over the mirrored pair: it is read at column W - 1 - x, and its normal
+// maps to (nx, -ny, nz) (the mirror flips the sign of the disparity's x
+// slope, the right view's plane convention flips both slopes).  A
+// non-finite or non-positive proposed disparity leaves the random plane.
+template <int kKind>
+__global__ void __launch_bounds__(256)
+k_init_proposal(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
+                const float* __restrict__ prop, int os, int Wp, float dscale,
+                float4* __restrict__ states) {
+  extern __shared__ float4 smem[];
+  const Lane L = block_setup<kKind == kCostColorGrad>(g, self, other, smem);
+  float4 q = mk4(0.0f, 0.0f, 0.0f, 0.0f), sp = q;
+  bool have = false;
+  if (L.valid) {
+    Mwc rng = mwc_jump((uint64_t)L.p * 3u);
+    const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
+    q = state_init(u0, u1, u2, L.xf, L.yf, lr, dmax);
+    const int xs = lr ? L.x : L.W - 1 - L.x;
+    const float* o = prop + ((size_t)L.y * Wp + xs) * os;
+    const float d = o[0] * dscale;
+    float3 n = make_float3(o[1], lr ? o[2] : -o[2], o[3]);
+    if (d > 0.0f && finite_f(d) && finite_f(n.x) && finite_f(n.y) &&
+        finite_f(n.z)) {
+      n = normalize3(n);
+      sp = from_disparity_normal(L.xf, L.yf, lr ? d : -d, n);
+      have = true;
+    }
+  }
+  const float cq = cost_any<kKind, false>(L, q, 1e10f, true, nullptr);
+  const float cp = cost_any<kKind, false>(L, sp, 1e10f, have, nullptr);
+  if (L.valid) {
+    q.w = cq;
+    sp.w = cp;
+    const bool keep = have && (cp <= cq || cq != cq);
+    states[L.p] = keep ? sp : q;
   }
 }
 
@@ -554,7 +682,7 @@ struct PropCtx {
 
 // candidate test with minimalCost (pmhstereo.cl:873-877) and the select
 // rule (pmhstereo.cl:976); called by the whole wavefront
-template <bool kCount>
+template <int kKind, bool kCount>
 __device__ __forceinline__ void try_candidate(const Lane& L, const PropCtx& c,
                                               float4& self, float4 cand,
                                               bool active, uint32_t* ntaps) {
@@ -565,7 +693,7 @@ __device__ __forceinline__ void try_candidate(const Lane& L, const PropCtx& c,
     dc = smooth_dist(cand, c.smooth, L.xf, L.yf, c.dmax, c.lr);
     mincost = (self.w + c.theta * ds) - c.theta * dc;
   }
-  const float w = cost_eval<kCount>(L, cand, mincost, active, ntaps);
+  const float w = cost_any<kKind, kCount>(L, cand, mincost, active, ntaps);
   if (active) {
     cand.w = w;
     if (cand.w + c.theta * dc <= self.w + c.theta * ds) self = cand;
@@ -615,11 +743,11 @@ __device__ __forceinline__ void scatter(float4 self, int x, int y, int W,
 // to scratch.  Measured on the 720p pair against the unconstrained build
 // (107 VGPRs, four wavefronts): 8 iterations 52.1 -> 51.8 ms; six
 // wavefronts (80 VGPRs, more spills) gain nothing.
-template <bool kCount>
+template <int kKind, bool kCount>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
 k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
   extern __shared__ float4 smem[];
-  const Lane L = block_setup(g, a.self, a.other, smem);
+  const Lane L = block_setup<kKind == kCostColorGrad>(g, a.self, a.other, smem);
   const int W = g.W, H = g.H;
   uint32_t ncost = 0;
   uint32_t* nc = kCount ? &ncost : nullptr;
@@ -637,10 +765,10 @@ k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
   c.smooth = sm;
 
   {
-    const float w = cost_eval<kCount>(L, self, 1e10f, true, nc);
+    const float w = cost_any<kKind, kCount>(L, self, 1e10f, true, nc);
     if (L.valid) self.w = w;
   }
-  Mwc rng = mwc_jump((uint64_t)(L.p + (size_t)a.iter) * kPropDraws);
+  Mwc rng = mwc_jump(prop_draw_start(L.p, a.iter));
 
   // smooth proposal (pmhstereo.cl:968-976)
   {
@@ -648,7 +776,7 @@ k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
         L.xf, L.yf, sm.w * (a.lr ? -1.0f : 1.0f),
         normalize3(make_float3(sm.x, sm.y, sm.z)));
     if (!a.lr) cand = viewprop(cand);
-    try_candidate<kCount>(L, c, self, cand, true, nc);
+    try_candidate<kKind, kCount>(L, c, self, cand, true, nc);
   }
   // random neighbours (pmhstereo.cl:981-991), x offset drawn first
   for (int i = 0; i < kNumRndSample; ++i) {
@@ -661,12 +789,12 @@ k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
       const int nx = clampi(L.x + ox, 0, W - 1), ny = clampi(L.y + oy, 0, H - 1);
       cand = a.old[(size_t)ny * W + nx];
     }
-    try_candidate<kCount>(L, c, self, cand, true, nc);
+    try_candidate<kKind, kCount>(L, c, self, cand, true, nc);
   }
   // random plane (pmhstereo.cl:996-1002)
   {
     const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
-    try_candidate<kCount>(L, c, self,
+    try_candidate<kKind, kCount>(L, c, self,
                           state_init(u0, u1, u2, L.xf, L.yf, a.lr, a.dmax),
                           true, nc);
   }
@@ -685,13 +813,13 @@ k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
         const int xs = a.lr ? (int)keys[i] : W - 1 - (int)keys[i];
         cand = viewprop(a.src_other[(size_t)L.y * W + xs]);
       }
-      try_candidate<kCount>(L, c, self, cand, have, nc);
+      try_candidate<kKind, kCount>(L, c, self, cand, have, nc);
     }
   }
   // randomized refinement (pmhstereo.cl:1021-1028)
   {
     const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
-    try_candidate<kCount>(
+    try_candidate<kKind, kCount>(
         L, c, self, state_refine(u0, u1, u2, self, L.xf, L.yf, a.dmax, a.lr),
         true, nc);
   }
@@ -699,7 +827,7 @@ k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
   {
     float4 k = self;
     for (int it = 0; it < kKltPasses; ++it) k = klt_pass(L, k);
-    try_candidate<kCount>(L, c, self, k, true, nc);
+    try_candidate<kKind, kCount>(L, c, self, k, true, nc);
   }
   if (!L.valid) return;
   if (a.viewout) scatter(self, L.x, L.y, W, a.lr, a.viewout);
@@ -727,13 +855,14 @@ k_scatter(int W, int H, const float4* __restrict__ states, int lr,
   scatter(states[(size_t)y * W + x], x, y, W, lr, viewout);
 }
 
+template <int kKind>
 __global__ void __launch_bounds__(256)
 k_cost_field(Geometry g, ViewBufs self, ViewBufs other,
              const float4* __restrict__ planes, float* __restrict__ cost) {
   extern __shared__ float4 smem[];
-  const Lane L = block_setup(g, self, other, smem);
+  const Lane L = block_setup<kKind == kCostColorGrad>(g, self, other, smem);
   const float4 s = L.valid ? planes[L.p] : mk4(0.0f, 0.0f, 0.0f, 0.0f);
-  const float w = cost_eval<false>(L, s, 1e10f, true, nullptr);
+  const float w = cost_any<kKind, false>(L, s, 1e10f, true, nullptr);
   if (L.valid) cost[L.p] = w;
 }
 
@@ -742,7 +871,7 @@ k_klt_field(Geometry g, ViewBufs self, ViewBufs other,
             const float4* __restrict__ planes, int passes,
             float4* __restrict__ out) {
   extern __shared__ float4 smem[];
-  const Lane L = block_setup(g, self, other, smem);
+  const Lane L = block_setup<false>(g, self, other, smem);
   float4 s = L.valid ? planes[L.p] : mk4(0.0f, 0.0f, 0.0f, 0.0f);
   for (int i = 0; i < passes; ++i) s = klt_pass(L, s);
   if (L.valid) out[L.p] = s;
@@ -918,9 +1047,9 @@ static hipError_t allow_lds(K kernel, size_t bytes) {
                              (int)bytes);
 }
 
-#define PM_LAUNCH_TILE(kernel, g, st, ...)                              \
+#define PM_LAUNCH_TILE(kernel, kind, g, st, ...)                        \
   do {                                                                  \
-    const size_t lds_ = block_lds_bytes((g).R);                         \
+    const size_t lds_ = block_lds_bytes((g).R, kind);                   \
     hipError_t e_ = allow_lds(kernel, lds_);                            \
     if (e_ != hipSuccess) return e_;                                    \
     PM_LAUNCH(kernel, grid_for(g), kBlockTile, lds_, st,                \
@@ -936,7 +1065,11 @@ hipError_t launch_prepare(hipStream_t st, const Geometry& g, const float* img,
 
 hipError_t launch_init(hipStream_t st, const Geometry& g, ViewBufs self,
                        ViewBufs other, float dmax, int lr, float4* states) {
-  PM_LAUNCH_TILE(k_init, g, st, g, self, other, dmax, lr, states);
+  if (g.cost_kind == kCostColorGrad)
+    PM_LAUNCH_TILE(k_init<kCostColorGrad>, kCostColorGrad, g, st, g, self,
+                   other, dmax, lr, states);
+  PM_LAUNCH_TILE(k_init<kCostCensus>, kCostCensus, g, st, g, self, other,
+                 dmax, lr, states);
 }
 
 hipError_t launch_disparity_to_normal(hipStream_t st, const Geometry& g,
@@ -950,8 +1083,22 @@ hipError_t launch_init_prior(hipStream_t st, const Geometry& g, ViewBufs self,
                              ViewBufs other, float dmax, int lr,
                              const float* disp, const float4* normals,
                              float4* states) {
-  PM_LAUNCH_TILE(k_init_prior, g, st, g, self, other, dmax, lr, disp, normals,
-                 states);
+  if (g.cost_kind == kCostColorGrad)
+    PM_LAUNCH_TILE(k_init_prior<kCostColorGrad>, kCostColorGrad, g, st, g,
+                   self, other, dmax, lr, disp, normals, states);
+  PM_LAUNCH_TILE(k_init_prior<kCostCensus>, kCostCensus, g, st, g, self,
+                 other, dmax, lr, disp, normals, states);
+}
+
+hipError_t launch_init_proposal(hipStream_t st, const Geometry& g,
+                                ViewBufs self, ViewBufs other, float dmax,
+                                int lr, const float* prop, int os, int Wp,
+                                float dscale, float4* states) {
+  if (g.cost_kind == kCostColorGrad)
+    PM_LAUNCH_TILE(k_init_proposal<kCostColorGrad>, kCostColorGrad, g, st, g,
+                   self, other, dmax, lr, prop, os, Wp, dscale, states);
+  PM_LAUNCH_TILE(k_init_proposal<kCostCensus>, kCostCensus, g, st, g, self,
+                 other, dmax, lr, prop, os, Wp, dscale, states);
 }
 
 hipError_t launch_viewbuf_clear_rows(hipStream_t st, const Geometry& g,
@@ -971,20 +1118,31 @@ hipError_t launch_viewbuf_clear(hipStream_t st, const Geometry& g,
   PM_LAUNCH(k_viewbuf_clear, dim3(blocks), dim3(256), 0, st, g.npix, vbuf);
 }
 
+template <int kKind>
+static hipError_t launch_propagate_kind(hipStream_t st, const Geometry& g,
+                                        const PropagateArgs& a,
+                                        unsigned long long* counters,
+                                        size_t lds_floor) {
+  const size_t lds = std::max(block_lds_bytes(g.R, kKind), lds_floor);
+  if (counters) {
+    const hipError_t e = allow_lds(k_propagate<kKind, true>, lds);
+    if (e != hipSuccess) return e;
+    PM_LAUNCH((k_propagate<kKind, true>), grid_for(g), kBlockTile, lds, st, g,
+              a, counters);
+  }
+  const hipError_t e = allow_lds(k_propagate<kKind, false>, lds);
+  if (e != hipSuccess) return e;
+  PM_LAUNCH((k_propagate<kKind, false>), grid_for(g), kBlockTile, lds, st, g,
+            a, counters);
+}
+
 hipError_t launch_propagate(hipStream_t st, const Geometry& g,
                             const PropagateArgs& a,
                             unsigned long long* counters, size_t lds_floor) {
-  const size_t lds = std::max(block_lds_bytes(g.R), lds_floor);
-  if (counters) {
-    const hipError_t e = allow_lds(k_propagate<true>, lds);
-    if (e != hipSuccess) return e;
-    PM_LAUNCH(k_propagate<true>, grid_for(g), kBlockTile, lds, st, g, a,
-              counters);
-  }
-  const hipError_t e = allow_lds(k_propagate<false>, lds);
-  if (e != hipSuccess) return e;
-  PM_LAUNCH(k_propagate<false>, grid_for(g), kBlockTile, lds, st, g, a,
-            counters);
+  if (g.cost_kind == kCostColorGrad)
+    return launch_propagate_kind<kCostColorGrad>(st, g, a, counters,
+                                                 lds_floor);
+  return launch_propagate_kind<kCostCensus>(st, g, a, counters, lds_floor);
 }
 
 hipError_t launch_scatter(hipStream_t st, const Geometry& g,
@@ -997,13 +1155,18 @@ hipError_t launch_scatter(hipStream_t st, const Geometry& g,
 hipError_t launch_cost_field(hipStream_t st, const Geometry& g, ViewBufs self,
                              ViewBufs other, const float4* planes,
                              float* cost) {
-  PM_LAUNCH_TILE(k_cost_field, g, st, g, self, other, planes, cost);
+  if (g.cost_kind == kCostColorGrad)
+    PM_LAUNCH_TILE(k_cost_field<kCostColorGrad>, kCostColorGrad, g, st, g,
+                   self, other, planes, cost);
+  PM_LAUNCH_TILE(k_cost_field<kCostCensus>, kCostCensus, g, st, g, self,
+                 other, planes, cost);
 }
 
 hipError_t launch_klt_field(hipStream_t st, const Geometry& g, ViewBufs self,
                             ViewBufs other, const float4* planes, int passes,
                             float4* out) {
-  PM_LAUNCH_TILE(k_klt_field, g, st, g, self, other, planes, passes, out);
+  PM_LAUNCH_TILE(k_klt_field, kCostCensus, g, st, g, self, other, planes,
+                 passes, out);
 }
 
 hipError_t launch_consistency(hipStream_t st, const Geometry& g,

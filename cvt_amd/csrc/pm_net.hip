@@ -1,0 +1,474 @@
+// pm_net.hip -- gfx950 kernels of the plane-proposal network: a 3x3
+// convolution on bf16 NHWC activations as an implicit GEMM on the matrix
+// cores (v_mfma_f32_16x16x32_bf16, fp32 accumulation), its weight gradient
+// (fp32 VALU), and the pooling, gradient routing, loss, Adam and input
+// packing kernels of a U-Net training step.
+#include <algorithm>
+
+#include "pm_net_kernels.h"
+
+namespace pm {
+namespace net {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+#define NET_LAUNCH(kernel, grid, block, lds, st, ...)               \
+  do {                                                              \
+    (void)hipGetLastError();                                        \
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__);  \
+    return hipGetLastError();                                       \
+  } while (0)
+
+__device__ __forceinline__ bf16x8 load8(const bf16* p) {
+  return *reinterpret_cast<const bf16x8*>(p);
+}
+
+__device__ __forceinline__ bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) z[i] = (bf16)0.0f;
+  return z;
+}
+
+// ---------------------------------------------------------------------
+// convolution: a work-group is one wavefront computing 64 output pixels x
+// 32 output channels as 4 x 2 tiles of 16 x 16.  Per K step of 32 a lane
+// holds the A operand of its tile row (pixel) and k-group (8 consecutive
+// k: one tap, 8 channels, one 16-byte load from NHWC) and the B operand
+// of its tile column (output channel: 8 consecutive k of its weight row).
+// The accumulator of a lane holds rows 4 g .. 4 g + 3, column lane % 16.
+// ---------------------------------------------------------------------
+constexpr int kConvPix = 64;
+constexpr int kConvCo = 32;
+
+__global__ void __launch_bounds__(64) k_conv(ConvArgs a) {
+  const int lane = threadIdx.x;
+  const int row = lane & 15, grp = lane >> 4;
+  const int cin = a.c1 + a.c2;
+  const int kpad = conv_kpad(cin), kreal = 9 * cin;
+  const long npix = (long)a.N * a.H * a.W;
+  const long p0 = (long)blockIdx.x * kConvPix;
+  const int co0 = blockIdx.y * kConvCo;
+  const int H2 = a.up2 ? (a.H + 1) >> 1 : a.H;
+  const int W2 = a.up2 ? (a.W + 1) >> 1 : a.W;
+  int pn[4], py[4], px[4];
+  bool pv[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const long p = p0 + mt * 16 + row;
+    pv[mt] = p < npix;
+    const long q = pv[mt] ? p : 0;
+    px[mt] = (int)(q % a.W);
+    const long t = q / a.W;
+    py[mt] = (int)(t % a.H);
+    pn[mt] = (int)(t / a.H);
+  }
+  f32x4 acc[4][2];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16* w0 = a.w + (size_t)(co0 + row) * kpad;
+  const bf16* w1 = a.w + (size_t)(co0 + 16 + row) * kpad;
+  // this lane's k = k0 + 8 grp as (tap, channel), advanced by 32 per step
+  int tap = (8 * grp) / cin, ci = (8 * grp) % cin;
+  for (int k0 = 0; k0 < kpad; k0 += 32) {
+    const int k = k0 + 8 * grp;
+    bf16x8 A[4];
+    if (k < kreal) {
+      const int dy = (tap / 3 - 1) * a.dil, dx = (tap % 3 - 1) * a.dil;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const int yy = py[mt] + dy, xx = px[mt] + dx;
+        A[mt] = zero8();
+        if (pv[mt] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+          if (ci < a.c1) {
+            A[mt] = load8(a.x1 +
+                          (((size_t)pn[mt] * a.H + yy) * a.W + xx) * a.c1 +
+                          ci);
+          } else {
+            const int y2 = a.up2 ? yy >> 1 : yy, x2 = a.up2 ? xx >> 1 : xx;
+            A[mt] = load8(a.x2 +
+                          (((size_t)pn[mt] * H2 + y2) * W2 + x2) * a.c2 +
+                          (ci - a.c1));
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) A[mt] = zero8();
+    }
+    const bf16x8 B0 = load8(w0 + k), B1 = load8(w1 + k);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      acc[mt][0] =
+          __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B0, acc[mt][0], 0, 0, 0);
+      acc[mt][1] =
+          __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B1, acc[mt][1], 0, 0, 0);
+    }
+    ci += 32;
+    while (ci >= cin) {
+      ci -= cin;
+      ++tap;
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+      const int co = co0 + nt * 16 + row;
+      if (co >= a.cout) continue;
+      const float b = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const long p = p0 + mt * 16 + 4 * grp + j;
+        if (p >= npix) continue;
+        float v = acc[mt][nt][j] + b;
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.yf)
+          a.yf[p * a.ys + co] = v;
+        else
+          a.y[p * a.ys + co] = (bf16)v;
+      }
+    }
+}
+
+hipError_t launch_conv(hipStream_t st, const ConvArgs& a) {
+  const long npix = (long)a.N * a.H * a.W;
+  const dim3 grid((unsigned)((npix + kConvPix - 1) / kConvPix),
+                  (unsigned)(conv_cpad(a.cout) / kConvCo));
+  NET_LAUNCH(k_conv, grid, dim3(64), 0, st, a);
+}
+
+// ---------------------------------------------------------------------
+// weight gradient: a work-group owns 32 k x 32 output channels and a chunk
+// of pixels; 32-pixel slices of the input patches and of the output
+// gradient are staged in LDS, each thread accumulates 4 channels of one k
+// ---------------------------------------------------------------------
+constexpr int kWgChunk = 1024;
+
+__global__ void __launch_bounds__(256)
+k_conv_wgrad(ConvArgs a, const bf16* __restrict__ g, float* __restrict__ dw,
+             float* __restrict__ db) {
+  __shared__ float sx[32][33];
+  __shared__ float sg[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int cin = a.c1 + a.c2, kpad = conv_kpad(cin), kreal = 9 * cin;
+  const int k = blockIdx.x * 32 + tx;
+  const int co0 = blockIdx.y * 32;
+  const long npix = (long)a.N * a.H * a.W;
+  const long pb0 = (long)blockIdx.z * kWgChunk;
+  const long pe = std::min<long>(npix, pb0 + kWgChunk);
+  const int H2 = a.up2 ? (a.H + 1) >> 1 : a.H;
+  const int W2 = a.up2 ? (a.W + 1) >> 1 : a.W;
+  // this thread stages k (its tx) of the patches and channel co0 + tx of g
+  const bool kin = k < kreal;
+  const int tap = kin ? k / cin : 0, ci = kin ? k - tap * cin : 0;
+  const int dy = (tap / 3 - 1) * a.dil, dx = (tap % 3 - 1) * a.dil;
+  const int gco = co0 + tx;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f}, accb[4] = {0.f, 0.f, 0.f, 0.f};
+  for (long pb = pb0; pb < pe; pb += 32) {
+    for (int pp = ty; pp < 32; pp += 8) {
+      const long p = pb + pp;
+      float xv = 0.0f, gv = 0.0f;
+      if (p < pe) {
+        const int xq = (int)(p % a.W);
+        const long t = p / a.W;
+        const int yq = (int)(t % a.H), nq = (int)(t / a.H);
+        const int yy = yq + dy, xx = xq + dx;
+        if (kin && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+          if (ci < a.c1) {
+            xv = (float)a.x1[(((size_t)nq * a.H + yy) * a.W + xx) * a.c1 + ci];
+          } else {
+            const int y2 = a.up2 ? yy >> 1 : yy, x2 = a.up2 ? xx >> 1 : xx;
+            xv = (float)a.x2[(((size_t)nq * H2 + y2) * W2 + x2) * a.c2 +
+                             (ci - a.c1)];
+          }
+        }
+        if (gco < a.cout) gv = (float)g[p * a.cout + gco];
+      }
+      sx[pp][tx] = xv;
+      sg[pp][tx] = gv;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int pp = 0; pp < 32; ++pp) {
+      const float xv = sx[pp][tx];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float gv = sg[pp][ty * 4 + j];
+        acc[j] = fmaf(gv, xv, acc[j]);
+        accb[j] += gv;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int co = co0 + ty * 4 + j;
+    if (co >= a.cout) continue;
+    if (kin) atomicAdd(&dw[(size_t)co * kpad + k], acc[j]);
+    if (db && blockIdx.x == 0 && tx == 0) atomicAdd(&db[co], accb[j]);
+  }
+}
+
+hipError_t launch_conv_wgrad(hipStream_t st, const ConvArgs& a, const bf16* g,
+                             float* dw, float* db) {
+  const long npix = (long)a.N * a.H * a.W;
+  const dim3 grid((unsigned)(conv_kpad(a.c1 + a.c2) / 32),
+                  (unsigned)(conv_cpad(a.cout) / 32),
+                  (unsigned)((npix + kWgChunk - 1) / kWgChunk));
+  NET_LAUNCH(k_conv_wgrad, grid, dim3(256), 0, st, a, g, dw, db);
+}
+
+// ---------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_flip_weights(const bf16* __restrict__ w, int cin, int cout,
+               bf16* __restrict__ wt) {
+  const int kpt = conv_kpad(cout), rows = conv_cpad(cin);
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)rows * kpt) return;
+  const int ci = (int)(i / kpt), kk = (int)(i % kpt);
+  bf16 v = (bf16)0.0f;
+  if (ci < cin && kk < 9 * cout) {
+    const int t = kk / cout, co = kk - t * cout;
+    v = w[(size_t)co * conv_kpad(cin) + (size_t)(8 - t) * cin + ci];
+  }
+  wt[i] = v;
+}
+
+hipError_t launch_flip_weights(hipStream_t st, const bf16* w, int cin,
+                               int cout, bf16* wt) {
+  const long n = (long)conv_cpad(cin) * conv_kpad(cout);
+  NET_LAUNCH(k_flip_weights, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+             st, w, cin, cout, wt);
+}
+
+__global__ void __launch_bounds__(256)
+k_grad_mask(size_t n, const float* __restrict__ g, const bf16* __restrict__ y,
+            int relu, bf16* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float v = g[i];
+  if (relu && !((float)y[i] > 0.0f)) v = 0.0f;
+  out[i] = (bf16)v;
+}
+
+hipError_t launch_grad_mask(hipStream_t st, size_t n, const float* g,
+                            const bf16* y, int relu, bf16* out) {
+  NET_LAUNCH(k_grad_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+             n, g, y, relu, out);
+}
+
+__global__ void __launch_bounds__(256)
+k_route_first(size_t npix, int c1, int c2, const float* __restrict__ t,
+              float* __restrict__ g1) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= npix * c1) return;
+  const size_t p = i / c1, c = i % c1;
+  g1[i] += t[p * (c1 + c2) + c];
+}
+
+__global__ void __launch_bounds__(256)
+k_route_second(int N, int H, int W, int c1, int c2, int up2,
+               const float* __restrict__ t, float* __restrict__ g2) {
+  const int H2 = up2 ? (H + 1) >> 1 : H, W2 = up2 ? (W + 1) >> 1 : W;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (size_t)N * H2 * W2 * c2) return;
+  const int c = (int)(i % c2);
+  const size_t q = i / c2;
+  const int x2 = (int)(q % W2);
+  const size_t r = q / W2;
+  const int y2 = (int)(r % H2), n = (int)(r / H2);
+  const int ct = c1 + c2;
+  float s = 0.0f;
+  if (up2) {
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const int y = 2 * y2 + dy, x = 2 * x2 + dx;
+        if (y < H && x < W)
+          s += t[(((size_t)n * H + y) * W + x) * ct + c1 + c];
+      }
+  } else {
+    s = t[(((size_t)n * H + y2) * W + x2) * ct + c1 + c];
+  }
+  g2[i] += s;
+}
+
+hipError_t launch_route_grad(hipStream_t st, int N, int H, int W, int c1,
+                             int c2, int up2, const float* t, float* g1,
+                             float* g2) {
+  const size_t npix = (size_t)N * H * W;
+  if (g1 && c1 > 0) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_route_first,
+                       dim3((unsigned)((npix * c1 + 255) / 256)), dim3(256),
+                       0, st, npix, c1, c2, t, g1);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (!g2 || c2 == 0) return hipSuccess;
+  const size_t H2 = up2 ? (H + 1) / 2 : H, W2 = up2 ? (W + 1) / 2 : W;
+  const size_t n2 = (size_t)N * H2 * W2 * c2;
+  NET_LAUNCH(k_route_second, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0,
+             st, N, H, W, c1, c2, up2, t, g2);
+}
+
+__global__ void __launch_bounds__(256)
+k_avgpool(int N, int H, int W, int C, const bf16* __restrict__ x,
+          bf16* __restrict__ y) {
+  const int Ho = H / 2, Wo = W / 2;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (size_t)N * Ho * Wo * C) return;
+  const int c = (int)(i % C);
+  const size_t q = i / C;
+  const int xo = (int)(q % Wo);
+  const size_t r = q / Wo;
+  const int yo = (int)(r % Ho), n = (int)(r / Ho);
+  const size_t b = (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
+  const float s = ((float)x[b] + (float)x[b + C]) +
+                  ((float)x[b + (size_t)W * C] + (float)x[b + (size_t)W * C + C]);
+  y[i] = (bf16)(0.25f * s);
+}
+
+hipError_t launch_avgpool(hipStream_t st, int N, int H, int W, int C,
+                          const bf16* x, bf16* y) {
+  const size_t n = (size_t)N * (H / 2) * (W / 2) * C;
+  NET_LAUNCH(k_avgpool, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+             N, H, W, C, x, y);
+}
+
+__global__ void __launch_bounds__(256)
+k_avgpool_grad(int N, int H, int W, int C, const float* __restrict__ gout,
+               float* __restrict__ gin) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (size_t)N * H * W * C) return;
+  const int c = (int)(i % C);
+  const size_t q = i / C;
+  const int x = (int)(q % W);
+  const size_t r = q / W;
+  const int y = (int)(r % H), n = (int)(r / H);
+  const int Ho = H / 2, Wo = W / 2;
+  gin[i] += 0.25f * gout[(((size_t)n * Ho + (y >> 1)) * Wo + (x >> 1)) * C + c];
+}
+
+hipError_t launch_avgpool_grad(hipStream_t st, int N, int H, int W, int C,
+                               const float* gout, float* gin) {
+  const size_t n = (size_t)N * H * W * C;
+  NET_LAUNCH(k_avgpool_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+             st, N, H, W, C, gout, gin);
+}
+
+__global__ void __launch_bounds__(256)
+k_l1_loss(size_t npix, const float* __restrict__ o, int os,
+          const float4* __restrict__ t, float wd, float wn, float inv_count,
+          float* __restrict__ g, float* __restrict__ loss) {
+  __shared__ float part[256];
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  float l = 0.0f;
+  if (p < npix) {
+    const float4 tv = t[p];
+    const float* op = o + p * os;
+    float* gp = g + p * os;
+    for (int c = 0; c < os; ++c) gp[c] = 0.0f;
+    if (tv.x >= 0.0f) {
+      const float d[4] = {op[0] - tv.x, op[1] - tv.y, op[2] - tv.z,
+                          op[3] - tv.w};
+      const float w[4] = {wd, wn, wn, wn};
+      for (int c = 0; c < 4; ++c) {
+        l += w[c] * fabsf(d[c]);
+        gp[c] = d[c] > 0.0f ? w[c] * inv_count
+                            : (d[c] < 0.0f ? -w[c] * inv_count : 0.0f);
+      }
+    }
+  }
+  part[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, part[0] * inv_count);
+}
+
+hipError_t launch_l1_loss(hipStream_t st, size_t npix, const float* o, int os,
+                          const float* t, float wd, float wn, float inv_count,
+                          float* g, float* loss) {
+  NET_LAUNCH(k_l1_loss, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0,
+             st, npix, o, os, reinterpret_cast<const float4*>(t), wd, wn,
+             inv_count, g, loss);
+}
+
+__global__ void __launch_bounds__(256)
+k_adam(size_t n, float* __restrict__ w, const float* __restrict__ g,
+       float* __restrict__ m, float* __restrict__ v, float lr, float b1,
+       float b2, float eps, float c1, float c2, bf16* __restrict__ wb) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  wb[i] = (bf16)wi;
+}
+
+hipError_t launch_adam(hipStream_t st, size_t n, float* w, const float* g,
+                       float* m, float* v, float lr, float b1, float b2,
+                       float eps, float c1, float c2, bf16* wb) {
+  NET_LAUNCH(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+             w, g, m, v, lr, b1, b2, eps, c1, c2, wb);
+}
+
+__global__ void __launch_bounds__(256)
+k_to_bf16(size_t n, const float* __restrict__ x, bf16* __restrict__ y) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = (bf16)x[i];
+}
+
+hipError_t launch_to_bf16(hipStream_t st, size_t n, const float* x, bf16* y) {
+  NET_LAUNCH(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+             n, x, y);
+}
+
+__global__ void __launch_bounds__(256)
+k_pack_pair(const float* __restrict__ left, const float* __restrict__ right,
+            int W, int H, int C, int stride, int Hp, int Wp, int mirror,
+            bf16* __restrict__ x) {
+  const int xo = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int yo = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (xo >= Wp || yo >= Hp) return;
+  const int ys = min(yo, H - 1);
+  int xs = min(xo, W - 1);
+  const float* a = left;
+  const float* b = right;
+  if (mirror) {
+    xs = W - 1 - xs;
+    a = right;
+    b = left;
+  }
+  bf16x8 v;
+  const float* pa = a + (size_t)ys * stride;
+  const float* pb = b + (size_t)ys * stride;
+  for (int c = 0; c < 4; ++c) {
+    v[c] = (bf16)(C == 1 ? pa[xs] : pa[4 * (size_t)xs + c]);
+    v[4 + c] = (bf16)(C == 1 ? pb[xs] : pb[4 * (size_t)xs + c]);
+  }
+  *reinterpret_cast<bf16x8*>(x + ((size_t)yo * Wp + xo) * 8) = v;
+}
+
+hipError_t launch_pack_pair(hipStream_t st, const float* left,
+                            const float* right, int W, int H, int channels,
+                            int stride, int Hp, int Wp, int mirror, bf16* x) {
+  const dim3 grid((unsigned)((Wp + 63) / 64), (unsigned)((Hp + 3) / 4));
+  NET_LAUNCH(k_pack_pair, grid, dim3(256), 0, st, left, right, W, H, channels,
+             stride, Hp, Wp, mirror, x);
+}
+
+}  // namespace net
+}  // namespace pm

@@ -15,7 +15,8 @@ import ctypes
 import numpy as np
 
 from . import _native
-from ._native import (PM_PP_LRCHECK, PM_PP_LRCHECK_FILL, PM_PP_NONE,  # noqa
+from ._native import (PM_COST_CENSUS, PM_COST_COLOR_GRAD,  # noqa
+                      PM_PP_LRCHECK, PM_PP_LRCHECK_FILL, PM_PP_NONE,
                       PM_SMOOTH_DEPTHMAP, PM_SMOOTH_HUBER, PM_SMOOTH_NONE,
                       PmParams)
 
@@ -55,13 +56,16 @@ class Context:
 
     def __init__(self, radius=5, disparity_max=64.0, color_weight=12.0,
                  smoothing=PM_SMOOTH_NONE, theta_min=1e-4,
-                 schedule_iterations=0, device=0, stream=None):
+                 schedule_iterations=0, device=0, stream=None,
+                 cost_kind=PM_COST_CENSUS, cost_weight=16.0):
         self._lib = _native.lib()
         p = PmParams()
         self._lib.pm_params_default(ctypes.byref(p))
         p.radius = int(radius)
         p.disparity_max = float(disparity_max)
         p.color_weight = float(color_weight)
+        p.cost_kind = int(cost_kind)
+        p.cost_weight = float(cost_weight)
         p.smoothing = int(smoothing)
         p.theta_min = float(theta_min)
         p.schedule_iterations = int(schedule_iterations)
@@ -139,6 +143,11 @@ class Context:
         self._check(self._lib.pm_setup_prior(self._h, _ptr(d),
                                              _ptr(n) if n is not None
                                              else None))
+
+    def setup_proposal(self, net):
+        """Setup seeded by a plane-proposal network (cvt_amd.net.ProposalNet)
+        with a per-pixel fallback to the random init (pm_setup_proposal)."""
+        self._check(self._lib.pm_setup_proposal(self._h, net.handle))
 
     # -- iterate -------------------------------------------------------------
     def iterate(self, n=1):

@@ -68,21 +68,47 @@ typedef enum pm_smoothing {
     PM_SMOOTH_DEPTHMAP = 2
 } pm_smoothing;
 
+/* Candidate cost: what the reference's `#define patch_eval`
+ * (pmhstereo.cl:753) selects.  The KLT step is patch_census_klt2 with
+ * either (the propagate calls it by name, pmhstereo.cl:1036-1038). */
+typedef enum pm_cost_kind {
+    /* patch_eval_census (pmhstereo.cl:661-736), the reference's choice:
+     * support-weighted truncated colour difference and gradient-sign
+     * census, with the early exit */
+    PM_COST_CENSUS = 0,
+    /* patch_eval_color_grad_weighted (pmhstereo.cl:510-565): weight
+     * exp(-|dot(I1(p) - I1(q), gray)| * cost_weight), tap cost 0.05 x the
+     * colour differences + 0.95 x the gradient differences, each
+     * truncated at 0.05; no early exit and no invalid-plane test; a patch
+     * with no tap inside the other image costs NaN, which is never
+     * accepted */
+    PM_COST_COLOR_GRAD = 1
+} pm_cost_kind;
+
 typedef struct pm_params {
     int radius;          /* patch half-width r: the reference's `patchsize`
                             (PMHuberStereo.h:47; loops -r..r,
                             pmhstereo.cl:685-687).  1..16, default 5 */
     float disparity_max; /* `depthmax`, > 0, default 64 */
     float color_weight;  /* support-weight gain, CENSUS_COLOR_WEIGHT = 12
-                            (pmhstereo.cl:47) */
+                            (pmhstereo.cl:47): bilateral weight sum, census
+                            cost and KLT step */
+    int cost_kind;       /* pm_cost_kind, default PM_COST_CENSUS */
+    float cost_weight;   /* PM_COST_COLOR_GRAD's support-weight gain: 16 in
+                            the reference (pmhstereo.cl:531); a gamma of
+                            Bleyer's exp(-dI / gamma) on 0-255 data is
+                            255 / gamma on [0, 1] data (gamma 10: 25.5) */
     int smoothing;       /* pm_smoothing, default PM_SMOOTH_NONE */
     float theta_min;     /* PM-Huber: first theta, 1e-4 as disparityMap
                             passes it (PMHuberStereo.cpp:92) */
     int schedule_iterations; /* PM-Huber: N of theta = (i / N)^4 + theta_min
-                            after iteration i (PMHuberStereo.cpp:575-576);
-                            0 = the iterations run so far plus the count of
-                            the pm_iterate call (one call after setup gives
-                            the reference's schedule) */
+                            after iteration i (PMHuberStereo.cpp:575-576),
+                            depthMap: of ((i - 3) / (N - 4))^3 + 1e-3
+                            (:315-318); 0 = the iterations run so far plus
+                            the count of the pm_iterate call (one call
+                            after setup gives the reference's schedule).
+                            Iterations past the schedule's end (split calls)
+                            keep theta at its end value */
     int device;          /* HIP device ordinal, default 0 */
     void* stream;        /* hipStream_t to run on, NULL = own stream */
 } pm_params;
@@ -156,8 +182,9 @@ int pm_get_planes(pm_ctx* ctx, int view, float* out);
  * when schedule_iterations is set. */
 int pm_set_planes(pm_ctx* ctx, int view, const float* in, int iteration);
 
-/* Cost of the given planes at every pixel of a view, mincost = 1e10
- * (patch_eval_census, pmhstereo.cl:661-736).  planes W x H x 4, cost W x H
+/* Cost of the given planes at every pixel of a view with the context's
+ * cost kind, mincost = 1e10 (patch_eval_census, pmhstereo.cl:661-736, or
+ * patch_eval_color_grad_weighted, :510-565).  planes W x H x 4, cost W x H
  * (host).  Does not modify the context's plane fields. */
 int pm_cost(pm_ctx* ctx, int view, const float* planes, float* cost);
 

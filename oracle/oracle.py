@@ -30,6 +30,7 @@ _SIGS = {
     "pmo_mwc_state": (None, [_u64, _p, _p]),
     "pmo_draws": (None, [_u64, _i, _p]),
     "pmo_draws_stepped": (None, [_u64, _i, _p]),
+    "pmo_prop_draw_start": (_u64, [_u64, _i]),
     "pmo_viewprop": (None, [_p, _p]),
     "pmo_transform": (_f, [_p, _f, _f]),
     "pmo_to_normal": (None, [_p, _p]),
@@ -38,18 +39,19 @@ _SIGS = {
     "pmo_state_refine": (None, [_f, _f, _f, _p, _f, _f, _f, _i, _p]),
     "pmo_to_ref_normal_depth": (None, [_p, _f, _f, _i, _p]),
     "pmo_prepare": (None, [_p, _i, _i, _i, _i, _i, _f, _p]),
-    "pmo_cost_at": (_f, [_p, _p, _i, _i, _i, _f, _i, _i, _p, _f]),
-    "pmo_cost_field": (None, [_p, _p, _i, _i, _i, _f, _p, _p]),
+    "pmo_cost_at": (_f, [_p, _p, _i, _i, _i, _f, _i, _f, _i, _i, _p, _f]),
+    "pmo_cost_field": (None, [_p, _p, _i, _i, _i, _f, _i, _f, _p, _p]),
     "pmo_klt_at": (None, [_p, _p, _i, _i, _i, _f, _i, _i, _p, _p]),
-    "pmo_init": (None, [_p, _p, _i, _i, _i, _f, _f, _i, _p]),
-    "pmo_init_prior": (None, [_p, _p, _i, _i, _i, _f, _f, _i, _p, _p, _p]),
+    "pmo_init": (None, [_p, _p, _i, _i, _i, _f, _i, _f, _f, _i, _p]),
+    "pmo_init_prior": (None, [_p, _p, _i, _i, _i, _f, _i, _f, _f, _i, _p, _p,
+                              _p]),
     "pmo_viewbuf_clear": (None, [_p, _i, _i]),
-    "pmo_propagate": (None, [_p, _p, _i, _i, _i, _f, _f, _f, _i, _i, _p, _p,
-                             _p, _p, _p, _p]),
+    "pmo_propagate": (None, [_p, _p, _i, _i, _i, _f, _i, _f, _f, _f, _i, _i,
+                             _p, _p, _p, _p, _p, _p]),
     "pmo_consistency": (None, [_p, _p, _i, _i, _f, _f, _p]),
     "pmo_to_disparity": (None, [_p, _i, _i, _f, _i, _p]),
-    "pmo_run": (_i, [_p, _p, _i, _i, _i, _i, _i, _f, _i, _f, _f, _p, _p, _p,
-                     _p]),
+    "pmo_run": (_i, [_p, _p, _i, _i, _i, _i, _i, _f, _i, _i, _f, _f, _f, _p,
+                     _p, _p, _p]),
     "pmo_scatter": (None, [_p, _i, _i, _i, _p]),
     "pmo_fill_depthmap": (None, [_p, _i, _i, _f, _p]),
     "pmo_fill_normalmap": (None, [_p, _i, _i, _p]),
@@ -70,6 +72,10 @@ _SIGS = {
 }
 
 COLOR_WEIGHT = 12.0  # CENSUS_COLOR_WEIGHT, pmhstereo.cl:47
+# candidate costs (the reference's `#define patch_eval`, pmhstereo.cl:753)
+COST_CENSUS = 0       # patch_eval_census, pmhstereo.cl:661-736
+COST_COLOR_GRAD = 1   # patch_eval_color_grad_weighted, pmhstereo.cl:510-565
+CG_WEIGHT = 16.0      # its support-weight factor, pmhstereo.cl:531
 
 
 def build(always=True):
@@ -132,6 +138,12 @@ def draws(k, n, stepped=False):
     fn = lib().pmo_draws_stepped if stepped else lib().pmo_draws
     fn(int(k), int(n), _ptr(out))
     return out
+
+
+def prop_draw_start(p, it):
+    """stream index of the first draw of the propagate at pixel p,
+    iteration it (RNG_init at pmhstereo.cl:942)"""
+    return int(lib().pmo_prop_draw_start(int(p), int(it)))
 
 
 def viewprop(s):
@@ -202,21 +214,24 @@ def prepare(img, radius, color_weight=COLOR_WEIGHT):
 
 
 def cost_at(tex1, tex2, radius, x, y, s, mincost=1e10,
-            color_weight=COLOR_WEIGHT):
+            color_weight=COLOR_WEIGHT, cost_kind=COST_CENSUS,
+            cost_weight=CG_WEIGHT):
     H, W = tex1.shape[:2]
     s = _f32(s, (4,))
     return lib().pmo_cost_at(_ptr(_f32(tex1)), _ptr(_f32(tex2)), W, H,
-                             int(radius), float(color_weight), int(x), int(y),
-                             _ptr(s), float(mincost))
+                             int(radius), float(color_weight), int(cost_kind),
+                             float(cost_weight), int(x), int(y), _ptr(s),
+                             float(mincost))
 
 
-def cost_field(tex1, tex2, radius, planes, color_weight=COLOR_WEIGHT):
+def cost_field(tex1, tex2, radius, planes, color_weight=COLOR_WEIGHT,
+               cost_kind=COST_CENSUS, cost_weight=CG_WEIGHT):
     H, W = tex1.shape[:2]
     planes = _f32(planes, (H, W, 4))
     out = np.zeros((H, W), np.float32)
     lib().pmo_cost_field(_ptr(_f32(tex1)), _ptr(_f32(tex2)), W, H,
-                         int(radius), float(color_weight), _ptr(planes),
-                         _ptr(out))
+                         int(radius), float(color_weight), int(cost_kind),
+                         float(cost_weight), _ptr(planes), _ptr(out))
     return out
 
 
@@ -229,21 +244,25 @@ def klt_at(tex1, tex2, radius, x, y, s, color_weight=COLOR_WEIGHT):
     return out
 
 
-def init(tex1, tex2, radius, dmax, lr, color_weight=COLOR_WEIGHT):
+def init(tex1, tex2, radius, dmax, lr, color_weight=COLOR_WEIGHT,
+         cost_kind=COST_CENSUS, cost_weight=CG_WEIGHT):
     H, W = tex1.shape[:2]
     out = np.zeros((H, W, 4), np.float32)
     lib().pmo_init(_ptr(_f32(tex1)), _ptr(_f32(tex2)), W, H, int(radius),
-                   float(color_weight), float(dmax), int(lr), _ptr(out))
+                   float(color_weight), int(cost_kind), float(cost_weight),
+                   float(dmax), int(lr), _ptr(out))
     return out
 
 
 def init_prior(tex1, tex2, radius, dmax, lr, disparity, normals,
-               color_weight=COLOR_WEIGHT):
+               color_weight=COLOR_WEIGHT, cost_kind=COST_CENSUS,
+               cost_weight=CG_WEIGHT):
     H, W = tex1.shape[:2]
     out = np.zeros((H, W, 4), np.float32)
     lib().pmo_init_prior(_ptr(_f32(tex1)), _ptr(_f32(tex2)), W, H,
-                         int(radius), float(color_weight), float(dmax),
-                         int(lr), _ptr(_f32(disparity, (H, W))),
+                         int(radius), float(color_weight), int(cost_kind),
+                         float(cost_weight), float(dmax), int(lr),
+                         _ptr(_f32(disparity, (H, W))),
                          _ptr(_f32(normals, (H, W, 4))), _ptr(out))
     return out
 
@@ -254,7 +273,8 @@ def viewbuf_empty(W, H):
 
 def propagate(tex1, tex2, radius, dmax, lr, it, old, viewin=None,
               src_other=None, smooth=None, theta=0.0, scatter=True,
-              color_weight=COLOR_WEIGHT):
+              color_weight=COLOR_WEIGHT, cost_kind=COST_CENSUS,
+              cost_weight=CG_WEIGHT):
     """One Jacobi propagate of one view; returns (new_states, viewout)."""
     H, W = tex1.shape[:2]
     out = np.zeros((H, W, 4), np.float32)
@@ -263,7 +283,8 @@ def propagate(tex1, tex2, radius, dmax, lr, it, old, viewin=None,
         viewin = np.ascontiguousarray(viewin, np.uint64)
         src_other = _f32(src_other, (H, W, 4))
     lib().pmo_propagate(_ptr(_f32(tex1)), _ptr(_f32(tex2)), W, H, int(radius),
-                        float(color_weight), float(dmax), float(theta),
+                        float(color_weight), int(cost_kind),
+                        float(cost_weight), float(dmax), float(theta),
                         int(lr), int(it), _ptr(_f32(old, (H, W, 4))),
                         _ptr(_f32(smooth) if smooth is not None else None),
                         _ptr(viewin), _ptr(src_other), _ptr(vout), _ptr(out))
@@ -287,7 +308,7 @@ def to_disparity(states, lr=1, scale=1.0):
 
 
 def run(left, right, radius, dmax, iters, maxdisp=1.0, maxdeg=180.0,
-        trace=False):
+        trace=False, cost_kind=COST_CENSUS, cost_weight=CG_WEIGHT):
     """Plain PatchMatch pipeline; returns dict of stateL, stateR, disp,
     and (if trace) cost_trace [iters, 2, H, W]."""
     left, W, H, C, stride = _img_args(left)
@@ -298,7 +319,8 @@ def run(left, right, radius, dmax, iters, maxdisp=1.0, maxdeg=180.0,
     disp = np.zeros((H, W), np.float32)
     tr = np.zeros((iters, 2, H, W), np.float32) if trace else None
     rc = lib().pmo_run(_ptr(left), _ptr(right), W, H, C, stride, int(radius),
-                       float(dmax), int(iters), float(maxdisp), float(maxdeg),
+                       float(dmax), int(iters), int(cost_kind),
+                       float(cost_weight), float(maxdisp), float(maxdeg),
                        _ptr(sL), _ptr(sR), _ptr(disp), _ptr(tr))
     if rc != 0:
         raise MemoryError("pmo_run failed")

@@ -46,7 +46,6 @@ typedef struct { float x, y, z; } f3;
 #define NUMRNDSAMPLE 4            /* pmhstereo.cl:34 */
 #define NUMRNDSAMPLERADIUS 5.0f   /* pmhstereo.cl:35 */
 #define VIEWSAMPLES 2             /* pmhstereo.cl:46 */
-#define PROP_DRAWS 14             /* pmhstereo.cl:942: (0 + 2) * 3 + 2 * 4 */
 #define KLT_PASSES 6              /* pmhstereo.cl:1036-1038 */
 #define SLOT_EMPTY 0xFFFFFFFFu
 
@@ -236,6 +235,18 @@ void pmo_draws_stepped(uint64_t k, int n, float* out)
 {
     mwc_t s = mwc_at(k);
     for (int i = 0; i < n; i++) out[i] = mwc_next(&s);
+}
+
+/* First draw of a propagate at pixel p, iteration iter.  The call is
+ * RNG_init(&rng, p + iter, <numdraws text>) (pmhstereo.cl:942) and the
+ * macro expands to MWC64X_Skip(x, (linpos) * numdraws) (RNG.cl:54-57)
+ * with the argument text unparenthesised:
+ *   (p + iter) * (((2*0+1)*(2*0+1) - 1) + 2) * 3 + 2 * 4
+ * = 6 (p + iter) + 8   (PROPSIZE 0, NUMRNDTRIES 2, NUMRNDSAMPLE 4,
+ * pmhstereo.cl:28,32-33).  Neighbouring pixels' 14-draw windows overlap. */
+uint64_t pmo_prop_draw_start(uint64_t p, int iter)
+{
+    return (p + (uint64_t)iter) * 6u + 8u;
 }
 
 /* ======================================================================
@@ -528,9 +539,22 @@ void pmo_prepare(const float* img, int W, int H, int channels, int stride,
 }
 
 /* ======================================================================
- * Cost (patch_eval_census, pmhstereo.cl:661-736) and KLT step
+ * Costs (patch_eval_census, pmhstereo.cl:661-736, and
+ * patch_eval_color_grad_weighted, :510-565) and KLT step
  * (patch_census_klt2, pmhstereo.cl:326-410)
  * ==================================================================== */
+
+#define COLORGRADALPHA 0.05f      /* pmhstereo.cl:39 */
+#define COLORMAXDIFF 0.05f        /* pmhstereo.cl:40 */
+#define GRADMAXDIFF 0.05f         /* pmhstereo.cl:41 */
+
+/* which patch_eval the kernels use, and its support-weight factors */
+typedef struct {
+    float cw;  /* census: CENSUS_COLOR_WEIGHT (pmhstereo.cl:47), also the
+                  bilateral weight sum's and the KLT step's */
+    int kind;  /* PMO_COST_CENSUS or PMO_COST_COLOR_GRAD */
+    float kw;  /* colour+gradient cost: 16 in the reference (:531) */
+} cost_model;
 
 /* CLK_FILTER_LINEAR + CLK_ADDRESS_CLAMP_TO_EDGE on integer row `row` at
  * column coordinate u (texel centres at integers).  Only r,g,b and the
@@ -565,8 +589,55 @@ static inline int popc4(uint32_t m)
            (int)((m >> 3) & 1);
 }
 
-static float cost_at(const float* tex1, const float* tex2, int W, int H,
-                     int R, float cw, int x, int y, f4 s, float mincost)
+/* patch_eval_color_grad_weighted, pmhstereo.cl:510-565: support weight
+ * exp(-|dot(I1(p) - I1(q), gray)| * kw) (16 in the reference), tap cost
+ * C = 0.05 sum_3 min(|I1(q) - I2(x')|, 0.05) + 0.95 sum_4 min(|G1(q) -
+ * G2(x')|, 0.05), result sum(w C) / sum(w).  No early exit and no
+ * invalid-plane test; a tap counts when its own pixel is in the image and
+ * x' lies in [0, W).  No qualifying tap gives 0 / 0 = NaN, which the
+ * accept rule's <= never takes.  A non-finite x' fails the range test
+ * here; the reference samples the image at it, which OpenCL leaves
+ * undefined. */
+static float cost_cg_at(const float* tex1, const float* tex2, int W, int H,
+                        int R, float kw, int x, int y, f4 s)
+{
+    const float xf = (float)x, yf = (float)y;
+    const float* tc = tex1 + ((size_t)y * W + x) * 8;
+    float ret = 0.0f, wsum = 0.0f;
+    for (int dy = -R; dy <= R; dy++) {
+        for (int dx = -R; dx <= R; dx++) {
+            float px = xf + (float)dx, py = yf + (float)dy;
+            if (px < 0.0f || px >= (float)W || py < 0.0f || py >= (float)H)
+                continue;
+            float xq = transform(s, px, py);
+            if (!(xq >= 0.0f && xq < (float)W)) continue;
+            int qx = x + dx, qy = y + dy;
+            const float* t1 = tex1 + ((size_t)qy * W + qx) * 8;
+            /* dot(valcenter - val1, gray): the negated difference gives
+             * the negated dot bit for bit, so |.| is the same */
+            float w = support_weight(t1[0] - tc[0], t1[1] - tc[1],
+                                     t1[2] - tc[2], kw);
+            float v2[8];
+            sample_row(tex2 + (size_t)qy * W * 8, W, xq, v2);
+            float sc = (fminf(fabsf(t1[0] - v2[0]), COLORMAXDIFF) +
+                        fminf(fabsf(t1[1] - v2[1]), COLORMAXDIFF)) +
+                       fminf(fabsf(t1[2] - v2[2]), COLORMAXDIFF);
+            float sg = ((fminf(fabsf(t1[4] - v2[4]), GRADMAXDIFF) +
+                         fminf(fabsf(t1[5] - v2[5]), GRADMAXDIFF)) +
+                        fminf(fabsf(t1[6] - v2[6]), GRADMAXDIFF)) +
+                       fminf(fabsf(t1[7] - v2[7]), GRADMAXDIFF);
+            /* alpha dot(.) + (1 - alpha) dot(.): the left product fused */
+            float C = fmaf(COLORGRADALPHA, sc, (1.0f - COLORGRADALPHA) * sg);
+            wsum += w;
+            ret = fmaf(w, C, ret);
+        }
+    }
+    return ret / wsum;
+}
+
+static float cost_census_at(const float* tex1, const float* tex2, int W,
+                            int H, int R, float cw, int x, int y, f4 s,
+                            float mincost)
 {
     const float xf = (float)x, yf = (float)y;
     float xp = transform(s, xf, yf);
@@ -604,6 +675,16 @@ static float cost_at(const float* tex1, const float* tex2, int W, int H,
     }
     if (wsum <= 1.0f) return 1.1f;
     return census / wsum;
+}
+
+/* the candidate cost the propagate's patch_eval selects */
+static float cost_at(const float* tex1, const float* tex2, int W, int H,
+                     int R, const cost_model* m, int x, int y, f4 s,
+                     float mincost)
+{
+    if (m->kind == PMO_COST_COLOR_GRAD)
+        return cost_cg_at(tex1, tex2, W, H, R, m->kw, x, y, s);
+    return cost_census_at(tex1, tex2, W, H, R, m->cw, x, y, s, mincost);
 }
 
 /* one census term of patch_census_klt2 at column u; val1 is the centred
@@ -688,22 +769,25 @@ static f4 klt_at(const float* tex1, const float* tex2, int W, int H, int R,
 }
 
 float pmo_cost_at(const float* tex1, const float* tex2, int W, int H,
-                  int radius, float cw, int x, int y, const float* s,
-                  float mincost)
+                  int radius, float cw, int kind, float kw, int x, int y,
+                  const float* s, float mincost)
 {
-    return cost_at(tex1, tex2, W, H, radius, cw, x, y,
+    cost_model m = {cw, kind, kw};
+    return cost_at(tex1, tex2, W, H, radius, &m, x, y,
                    mk4(s[0], s[1], s[2], s[3]), mincost);
 }
 
 void pmo_cost_field(const float* tex1, const float* tex2, int W, int H,
-                    int radius, float cw, const float* planes, float* cost)
+                    int radius, float cw, int kind, float kw,
+                    const float* planes, float* cost)
 {
+    cost_model m = {cw, kind, kw};
 #pragma omp parallel for schedule(dynamic, 1)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
             const float* s = planes + ((size_t)y * W + x) * 4;
             cost[(size_t)y * W + x] =
-                cost_at(tex1, tex2, W, H, radius, cw, x, y,
+                cost_at(tex1, tex2, W, H, radius, &m, x, y,
                         mk4(s[0], s[1], s[2], s[3]), 1e10f);
         }
 }
@@ -723,8 +807,10 @@ void pmo_klt_at(const float* tex1, const float* tex2, int W, int H,
 
 /* pmhstereo_init, pmhstereo.cl:755-773: RNG_init(p, 3) */
 void pmo_init(const float* tex1, const float* tex2, int W, int H, int radius,
-              float cw, float dmax, int lr, float* states)
+              float cw, int kind, float kw, float dmax, int lr,
+              float* states)
 {
+    cost_model m = {cw, kind, kw};
 #pragma omp parallel for schedule(dynamic, 1)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
@@ -733,7 +819,7 @@ void pmo_init(const float* tex1, const float* tex2, int W, int H, int radius,
             float u0 = mwc_next(&rng), u1 = mwc_next(&rng),
                   u2 = mwc_next(&rng);
             f4 s = state_init(u0, u1, u2, (float)x, (float)y, lr, dmax);
-            s.w = cost_at(tex1, tex2, W, H, radius, cw, x, y, s, 1e10f);
+            s.w = cost_at(tex1, tex2, W, H, radius, &m, x, y, s, 1e10f);
             memcpy(states + p * 4, &s, sizeof s);
         }
 }
@@ -742,10 +828,11 @@ void pmo_init(const float* tex1, const float* tex2, int W, int H, int radius,
  * disparity > 0 seeds the plane (its sign flipped for the right view),
  * elsewhere the random plane of pmhstereo_init. */
 void pmo_init_prior(const float* tex1, const float* tex2, int W, int H,
-                    int radius, float cw, float dmax, int lr,
-                    const float* disparity, const float* normals,
+                    int radius, float cw, int kind, float kw, float dmax,
+                    int lr, const float* disparity, const float* normals,
                     float* states)
 {
+    cost_model m = {cw, kind, kw};
 #pragma omp parallel for schedule(dynamic, 1)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
@@ -764,7 +851,7 @@ void pmo_init_prior(const float* tex1, const float* tex2, int W, int H,
                       u2 = mwc_next(&rng);
                 s = state_init(u0, u1, u2, (float)x, (float)y, lr, dmax);
             }
-            s.w = cost_at(tex1, tex2, W, H, radius, cw, x, y, s, 1e10f);
+            s.w = cost_at(tex1, tex2, W, H, radius, &m, x, y, s, 1e10f);
             memcpy(states + p * 4, &s, sizeof s);
         }
 }
@@ -823,7 +910,8 @@ static inline f4 load4(const float* a, size_t p)
 typedef struct {
     const float *tex1, *tex2;
     int W, H, R;
-    float cw, dmax, theta;
+    const cost_model* m;
+    float dmax, theta;
     int lr;
     float xf, yf;
     f4 smooth;
@@ -836,31 +924,32 @@ static void try_candidate(const prop_ctx* c, f4* self, f4 cand)
     float ds = smooth_dist(*self, c->smooth, c->xf, c->yf, c->dmax, c->lr);
     float dc = smooth_dist(cand, c->smooth, c->xf, c->yf, c->dmax, c->lr);
     float mincost = (self->w + c->theta * ds) - c->theta * dc;
-    cand.w = cost_at(c->tex1, c->tex2, c->W, c->H, c->R, c->cw, c->x, c->y,
+    cand.w = cost_at(c->tex1, c->tex2, c->W, c->H, c->R, c->m, c->x, c->y,
                      cand, mincost);
     if (cand.w + c->theta * dc <= self->w + c->theta * ds) *self = cand;
 }
 
 void pmo_propagate(const float* tex1, const float* tex2, int W, int H,
-                   int radius, float cw, float dmax, float theta, int lr,
-                   int iter, const float* old, const float* smooth,
-                   const uint64_t* viewin, const float* src_other,
-                   uint64_t* viewout, float* out)
+                   int radius, float cw, int kind, float kw, float dmax,
+                   float theta, int lr, int iter, const float* old,
+                   const float* smooth, const uint64_t* viewin,
+                   const float* src_other, uint64_t* viewout, float* out)
 {
+    const cost_model m = {cw, kind, kw};
     /* rows are independent: the scatter never leaves the row, so each
      * row's inserts run in one thread in x order */
 #pragma omp parallel for schedule(dynamic, 1)
     for (int y = 0; y < H; y++) {
         for (int x = 0; x < W; x++) {
             size_t p = (size_t)y * W + x;
-            prop_ctx c = {tex1, tex2, W, H, radius, cw, dmax, theta, lr,
+            prop_ctx c = {tex1, tex2, W, H, radius, &m, dmax, theta, lr,
                           (float)x, (float)y, mk4(0, 0, 0, 0), x, y};
             f4 sm = smooth ? load4(smooth, p) : mk4(0, 0, 0, 0);
             sm.w = fminf(fmaxf(sm.w, 0.0f), 1.0f) * dmax;
             c.smooth = sm;
             f4 self = load4(old, p);
-            self.w = cost_at(tex1, tex2, W, H, radius, cw, x, y, self, 1e10f);
-            mwc_t rng = mwc_at((uint64_t)(p + (size_t)iter) * PROP_DRAWS);
+            self.w = cost_at(tex1, tex2, W, H, radius, &m, x, y, self, 1e10f);
+            mwc_t rng = mwc_at(pmo_prop_draw_start(p, iter));
 
             /* smooth proposal, pmhstereo.cl:968-976 */
             f3 sn = {sm.x, sm.y, sm.z};
@@ -1363,8 +1452,8 @@ int pmo_num_threads(void)
  * then checkConsistencyLR + convertDisparity (PMHuberStereo.cpp:129-134). */
 int pmo_run(const float* left, const float* right, int W, int H,
             int channels, int stride, int radius, float dmax, int iters,
-            float maxdispdiff, float maxdeg, float* stateL, float* stateR,
-            float* disp, float* cost_trace)
+            int kind, float kw, float maxdispdiff, float maxdeg,
+            float* stateL, float* stateR, float* disp, float* cost_trace)
 {
     const float cw = 12.0f;
     size_t n = (size_t)W * H;
@@ -1380,16 +1469,20 @@ int pmo_run(const float* left, const float* right, int W, int H,
         return -1;
     pmo_prepare(left, W, H, channels, stride, radius, cw, texL);
     pmo_prepare(right, W, H, channels, stride, radius, cw, texR);
-    pmo_init(texL, texR, W, H, radius, cw, dmax, 1, sL[0]);
-    pmo_init(texR, texL, W, H, radius, cw, dmax, 0, sR[0]);
+    pmo_init(texL, texR, W, H, radius, cw, kind, kw,
+             dmax, 1, sL[0]);
+    pmo_init(texR, texL, W, H, radius, cw, kind, kw,
+             dmax, 0, sR[0]);
     pmo_viewbuf_clear(vb2, W, H);
     int cur = 0;
     for (int it = 0; it < iters; it++) {
         pmo_viewbuf_clear(vb1, W, H);
-        pmo_propagate(texL, texR, W, H, radius, cw, dmax, 0.0f, 1, it,
+        pmo_propagate(texL, texR, W, H, radius, cw, kind, kw,
+                      dmax, 0.0f, 1, it,
                       sL[cur], NULL, vb2, sR[cur], vb1, sL[1 - cur]);
         pmo_viewbuf_clear(vb2, W, H);
-        pmo_propagate(texR, texL, W, H, radius, cw, dmax, 0.0f, 0, it,
+        pmo_propagate(texR, texL, W, H, radius, cw, kind, kw,
+                      dmax, 0.0f, 0, it,
                       sR[cur], NULL, vb1, sL[1 - cur], vb2, sR[1 - cur]);
         cur = 1 - cur;
         if (cost_trace) {
@@ -1449,8 +1542,10 @@ int pmo_run_huber(const float* left, const float* right, int W, int H,
     pmo_prepare(right, W, H, channels, stride, radius, cw, texR);
     pmo_diffusion_tensor(left, W, H, channels, stride, 24.0f, dtL);
     pmo_diffusion_tensor(right, W, H, channels, stride, 24.0f, dtR);
-    pmo_init(texL, texR, W, H, radius, cw, dmax, 1, sL[0]);
-    pmo_init(texR, texL, W, H, radius, cw, dmax, 0, sR[0]);
+    pmo_init(texL, texR, W, H, radius, cw, PMO_COST_CENSUS,
+             16.0f, dmax, 1, sL[0]);
+    pmo_init(texR, texL, W, H, radius, cw, PMO_COST_CENSUS,
+             16.0f, dmax, 0, sR[0]);
     pmo_viewbuf_clear(vb2, W, H);
     float theta = theta_min;
     int cur = 0;
@@ -1458,10 +1553,12 @@ int pmo_run_huber(const float* left, const float* right, int W, int H,
         const float lam = theta * 10.0f + 1.0f;
         const float lambda[4] = {lam, lam, lam, 1.0f};
         pmo_viewbuf_clear(vb1, W, H);
-        pmo_propagate(texL, texR, W, H, radius, cw, dmax, theta * 50.0f, 1,
+        pmo_propagate(texL, texR, W, H, radius, cw, PMO_COST_CENSUS,
+                      16.0f, dmax, theta * 50.0f, 1,
                       it, sL[cur], smL, vb2, sR[cur], vb1, sL[1 - cur]);
         pmo_viewbuf_clear(vb2, W, H);
-        pmo_propagate(texR, texL, W, H, radius, cw, dmax, theta * 50.0f, 0,
+        pmo_propagate(texR, texL, W, H, radius, cw, PMO_COST_CENSUS,
+                      16.0f, dmax, theta * 50.0f, 0,
                       it, sR[cur], smR, vb1, sL[1 - cur], vb2, sR[1 - cur]);
         pmo_normal_depth(sL[1 - cur], W, H, 1.0f / dmax, 1, nd);
         if (pmo_rof(nd, dtL, lambda, 40, W, H, smL)) return -1;
@@ -1531,8 +1628,10 @@ int pmo_run_depthmap(const float* left, const float* right, int W, int H,
     pmo_prepare(right, W, H, channels, stride, radius, cw, texR);
     pmo_diffusion_tensor(left, W, H, channels, stride, 12.0f, dtL);
     pmo_diffusion_tensor(right, W, H, channels, stride, 12.0f, dtR);
-    pmo_init(texL, texR, W, H, radius, cw, dmax, 1, sL[0]);
-    pmo_init(texR, texL, W, H, radius, cw, dmax, 0, sR[0]);
+    pmo_init(texL, texR, W, H, radius, cw, PMO_COST_CENSUS,
+             16.0f, dmax, 1, sL[0]);
+    pmo_init(texR, texL, W, H, radius, cw, PMO_COST_CENSUS,
+             16.0f, dmax, 0, sR[0]);
     pmo_viewbuf_clear(vb2, W, H);
     float theta = 1e-3f;
     int cur = 0;
@@ -1540,10 +1639,12 @@ int pmo_run_depthmap(const float* left, const float* right, int W, int H,
         const float lam = theta * 15.0f + 2.0f;
         const float lambda[4] = {lam, lam, lam, lam};
         pmo_viewbuf_clear(vb1, W, H);
-        pmo_propagate(texL, texR, W, H, radius, cw, dmax, theta * 10.0f, 1,
+        pmo_propagate(texL, texR, W, H, radius, cw, PMO_COST_CENSUS,
+                      16.0f, dmax, theta * 10.0f, 1,
                       it, sL[cur], smL, vb2, sR[cur], vb1, sL[1 - cur]);
         pmo_viewbuf_clear(vb2, W, H);
-        pmo_propagate(texR, texL, W, H, radius, cw, dmax, theta * 10.0f, 0,
+        pmo_propagate(texR, texL, W, H, radius, cw, PMO_COST_CENSUS,
+                      16.0f, dmax, theta * 10.0f, 0,
                       it, sR[cur], smR, vb1, sL[1 - cur], vb2, sR[1 - cur]);
         pmo_fill_state(sL[1 - cur], W, H, 1.0f / dmax, 1, nd);
         if (pmo_huber(nd, dtL, lambda, 1e-4f, 40, W, H, smL)) return -1;

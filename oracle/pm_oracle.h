@@ -50,6 +50,9 @@ void pmo_mwc_state(uint64_t k, uint32_t* x, uint32_t* c);
 void pmo_draws(uint64_t k, int n, float* out);
 /* n draws starting at k produced by Skip(k) then n Step()s */
 void pmo_draws_stepped(uint64_t k, int n, float* out);
+/* stream index of the first draw of a propagate at pixel p, iteration
+ * iter: 6 (p + iter) + 8, the expansion of RNG_init at pmhstereo.cl:942 */
+uint64_t pmo_prop_draw_start(uint64_t p, int iter);
 
 /* ---- plane algebra (pmhstereo.cl:77-224) -------------------------------*/
 void pmo_viewprop(const float* s, float* out);
@@ -69,33 +72,44 @@ void pmo_to_ref_normal_depth(const float* s, float x, float y, int lr,
 void pmo_prepare(const float* img, int W, int H, int channels, int stride,
                  int radius, float color_weight, float* tex);
 
-/* ---- per-pixel kernels -------------------------------------------------*/
-/* support-weighted census cost of plane s at (x, y) (pmhstereo.cl:661-736) */
+/* ---- per-pixel kernels -------------------------------------------------
+ * Candidate costs: cost_kind selects what the reference's
+ * `#define patch_eval` (pmhstereo.cl:753) names.  color_weight is
+ * CENSUS_COLOR_WEIGHT (pmhstereo.cl:47: the bilateral weight sum, the
+ * census cost and the KLT step), cost_weight the colour+gradient cost's
+ * support-weight factor (16 in pmhstereo.cl:531). */
+#define PMO_COST_CENSUS 0      /* patch_eval_census, pmhstereo.cl:661-736 */
+#define PMO_COST_COLOR_GRAD 1  /* patch_eval_color_grad_weighted, :510-565 */
+/* cost of plane s at (x, y); mincost is the census early-exit bound */
 float pmo_cost_at(const float* tex1, const float* tex2, int W, int H,
-                  int radius, float color_weight, int x, int y,
-                  const float* s, float mincost);
+                  int radius, float color_weight, int cost_kind,
+                  float cost_weight, int x, int y, const float* s,
+                  float mincost);
 /* cost of every pixel's plane with mincost = 1e10 (no early exit) */
 void pmo_cost_field(const float* tex1, const float* tex2, int W, int H,
-                    int radius, float color_weight, const float* planes,
-                    float* cost);
+                    int radius, float color_weight, int cost_kind,
+                    float cost_weight, const float* planes, float* cost);
 /* one patch_census_klt2 pass (pmhstereo.cl:326-410) */
 void pmo_klt_at(const float* tex1, const float* tex2, int W, int H,
                 int radius, float color_weight, int x, int y,
                 const float* s, float* out);
 /* pmhstereo_init over the whole view (pmhstereo.cl:755-773) */
 void pmo_init(const float* tex1, const float* tex2, int W, int H, int radius,
-              float color_weight, float dmax, int lr, float* states);
+              float color_weight, int cost_kind, float cost_weight,
+              float dmax, int lr, float* states);
 /* pmhstereo_init_disparity_normal (pmhstereo.cl:802-828) */
 void pmo_init_prior(const float* tex1, const float* tex2, int W, int H,
-                    int radius, float color_weight, float dmax, int lr,
+                    int radius, float color_weight, int cost_kind,
+                    float cost_weight, float dmax, int lr,
                     const float* disparity, const float* normals,
                     float* states);
 void pmo_viewbuf_clear(uint64_t* vbuf, int W, int H);
 /* pmhstereo_propagate_view, Jacobi schedule (pmhstereo.cl:880-1071).
  * src_other: the other view's state field the keys in viewin refer to. */
 void pmo_propagate(const float* tex1, const float* tex2, int W, int H,
-                   int radius, float color_weight, float dmax, float theta,
-                   int lr, int iter, const float* old, const float* smooth,
+                   int radius, float color_weight, int cost_kind,
+                   float cost_weight, float dmax, float theta, int lr,
+                   int iter, const float* old, const float* smooth,
                    const uint64_t* viewin, const float* src_other,
                    uint64_t* viewout, float* out);
 /* the view scatter alone (inserts into vbuf, which is not cleared) */
@@ -124,8 +138,9 @@ void pmo_disparity_to_normal(const float* disp, int W, int H,
  * layout [iter][view][H][W]. */
 int pmo_run(const float* left, const float* right, int W, int H,
             int channels, int stride, int radius, float dmax, int iters,
-            float maxdispdiff, float maxdeg, float* stateL, float* stateR,
-            float* disp, float* cost_trace);
+            int cost_kind, float cost_weight, float maxdispdiff,
+            float maxdeg, float* stateL, float* stateR, float* disp,
+            float* cost_trace);
 
 /* ---- PM-Huber smoothing coupling (theta > 0) ---------------------------
  * PDDiffTensorColor / PDDiffTensorGray (cvt/cl/kernel/PDDiffTensor.cl:

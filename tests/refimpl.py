@@ -4,6 +4,7 @@ brute-force comparison with the oracle on tiny images:
 * gradxy (pmhstereo.cl:1549-1602) and bilateral_weight_to_alpha
   (pmhstereo.cl:1462-1498), with math.exp;
 * patch_eval_census (pmhstereo.cl:661-736), per tap with the early exit;
+* patch_eval_color_grad_weighted (pmhstereo.cl:510-565);
 * patch_census_klt2 (pmhstereo.cl:326-410) + mat3f_lusolve
   (Matrix3.cl:286-311).
 
@@ -126,6 +127,42 @@ def cost(tex1, tex2, r, x, y, s, mincost=1e10, cw=12.0):
     if wsum <= 1.0:
         return f32(1.1)
     return f32(census / wsum)
+
+
+def cost_cg(tex1, tex2, r, x, y, s, kw=16.0):
+    """patch_eval_color_grad_weighted: w1 = exp(-|dot(valcenter - val1,
+    gray)| * kw); C = 0.05 dot(fmin(|val1 - val2|, 0.05), 1) + (1 - 0.05)
+    dot(fmin(|gval1 - gval2|, 0.05), 1); ret1 / wsum1.  The tap loop's
+    tests are the kernel's; a NaN x' skips the tap (the oracle's stated
+    choice where the reference samples at a NaN coordinate)."""
+    H, W = tex1.shape[:2]
+    s = [f32(v) for v in s]
+    c = tex1[y, x]
+    ret, wsum = f32(0.0), f32(0.0)
+    alpha = f32(0.05)
+    with np.errstate(all="ignore"):
+        for dy in range(-r, r + 1):
+            for dx in range(-r, r + 1):
+                qx, qy = x + dx, y + dy
+                if qx < 0 or qx >= W or qy < 0 or qy >= H:
+                    continue
+                t1 = tex1[qy, qx]
+                d = [f32(c[k] - t1[k]) for k in range(3)]
+                g = fma(d[2], GW[2], fma(d[1], GW[1], f32(d[0] * GW[0])))
+                w1 = f32(math.exp(float(f32(-abs(g) * f32(kw)))))
+                xq = transform(s, qx, qy)
+                if not (xq >= 0 and xq < W):
+                    continue
+                wsum = f32(wsum + w1)
+                v2 = sample(tex2[qy], xq)
+                dc = [min(abs(f32(t1[k] - v2[k])), f32(0.05)) for k in range(3)]
+                dg = [min(abs(f32(t1[k] - v2[k])), f32(0.05))
+                      for k in range(4, 8)]
+                sc = f32(f32(dc[0] + dc[1]) + dc[2])
+                sg = f32(f32(f32(dg[0] + dg[1]) + dg[2]) + dg[3])
+                C = fma(alpha, sc, f32(f32(f32(1.0) - alpha) * sg))
+                ret = fma(w1, C, ret)
+        return f32(ret / wsum) if wsum != 0 else f32("nan")
 
 
 def klt_term(row2, u, val1, ga, w1):

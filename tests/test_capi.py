@@ -1,6 +1,6 @@
 """The C ABI library without a GPU: it loads, exports every entry point
-include/pm_stereo.h and include/pm_io.h declare, its headers compile as C
-and C++, parameters
+include/pm_stereo.h, include/pm_io.h and include/pm_net.h declare, its
+headers compile as C and C++, parameters
 are validated before any device is touched, and a host without a HIP
 device gets PM_ERR_HIP (no CPU fallback)."""
 import ctypes
@@ -15,12 +15,13 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pm_stereo.h")
 IO_HEADER = os.path.join(ROOT, "include", "pm_io.h")
+NET_HEADER = os.path.join(ROOT, "include", "pm_net.h")
 CXX_HEADER = os.path.join(ROOT, "include", "cvt_amd", "PMHuberStereo.h")
 
 
 def declared_functions():
     names = set()
-    for h in (HEADER, IO_HEADER):
+    for h in (HEADER, IO_HEADER, NET_HEADER):
         text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
         names |= set(re.findall(r"\b(pm_[a-z_0-9]+)\s*\(", text))
     return names
@@ -57,6 +58,8 @@ def test_params_default_are_reference_constants(native):
     assert p.radius == 5                 # cvt::PMHuberStereo patchsize
     assert p.disparity_max == 64.0
     assert p.color_weight == 12.0        # CENSUS_COLOR_WEIGHT
+    assert p.cost_kind == native.PM_COST_CENSUS     # patch_eval_census
+    assert p.cost_weight == 16.0         # patch_eval_color_grad_weighted
     assert p.smoothing == native.PM_SMOOTH_NONE     # plain PatchMatch
     assert p.theta_min == np.float32(1e-4)          # PMHuberStereo.cpp:92
     assert p.schedule_iterations == 0
@@ -67,7 +70,9 @@ def test_params_default_are_reference_constants(native):
     ("radius", 0), ("radius", 17), ("disparity_max", 0.0),
     ("disparity_max", float("nan")), ("disparity_max", float("inf")),
     ("color_weight", -1.0), ("theta_min", -0.5), ("theta_min", float("nan")),
-    ("smoothing", 3), ("schedule_iterations", -1), ("device", -1)])
+    ("smoothing", 3), ("schedule_iterations", -1), ("device", -1),
+    ("cost_kind", 2), ("cost_kind", -1), ("cost_weight", -1.0),
+    ("cost_weight", float("inf"))])
 def test_create_rejects_bad_parameters(native, field, value):
     L = native.lib()
     p = native.PmParams()
@@ -127,10 +132,12 @@ def test_missing_library_raises(native, monkeypatch):
 C_PROGRAM = r"""
 #include "pm_stereo.h"
 #include "pm_io.h"
+#include "pm_net.h"
 #include <stdio.h>
 int main(void) {
     pm_params p;
     pm_ctx* ctx = 0;
+    pm_net* net = 0;
     int n = 0;
     pm_params_default(&p);
     p.radius = 0;
@@ -138,6 +145,10 @@ int main(void) {
     printf("%d %d %s\n", p.disparity_max == 64.0f, rc, pm_last_error(ctx));
     pm_destroy(ctx);
     if (pm_default_histogram(0, NULL, &n) != PM_OK || n != 15) return 2;
+    /* no HIP device on this host: the net reports it, nothing runs */
+    if (pm_net_create(0, NULL, 1, &net) != PM_ERR_HIP) return 3;
+    pm_net_destroy(net);
+    if (pm_setup_proposal(NULL, NULL) != PM_ERR_ARG) return 4;
     return rc == PM_ERR_ARG ? 0 : 1;
 }
 """

@@ -115,6 +115,34 @@ def test_schedule_over_split_calls(pair, smoothing, n, split):
     assert_same(out[1][1], out[0][1], "smooth")
 
 
+@pytest.mark.parametrize("smoothing", [cvt_amd.PM_SMOOTH_HUBER,
+                                       cvt_amd.PM_SMOOTH_DEPTHMAP])
+def test_split_calls_past_the_schedule(pair, smoothing):
+    """Six calls of one iteration with schedule_iterations = 0 (each call's
+    schedule ends with it) run past the schedule's end, where the
+    reference never uses theta: theta holds at the end value instead of
+    dividing by zero (depthMap's N - 4 at N = 4).  The fields stay finite;
+    for depthMap they equal one call of six with schedule_iterations = 4,
+    whose thetas after iterations 3 and 4 are the same held end value."""
+    L, R, _ = pair
+    out = []
+    for sched, calls in ((0, (1,) * 6), (4, (6,))):
+        with cvt_amd.Context(radius=3, disparity_max=20.0,
+                             smoothing=smoothing,
+                             schedule_iterations=sched) as ctx:
+            ctx.set_images(L, R)
+            ctx.setup()
+            for k in calls:
+                ctx.iterate(k)
+            out.append((ctx.planes(0), ctx.planes(1), ctx.smooth(0),
+                        ctx.smooth(1)))
+    for a in out[0]:
+        assert np.isfinite(a[..., :3]).all()
+    if smoothing == cvt_amd.PM_SMOOTH_DEPTHMAP:
+        for a, b in zip(out[0], out[1]):
+            assert_same(a, b, "split calls past the schedule")
+
+
 def test_prior_seeds_the_smooth_field(oracle, pair):
     L, R, gt = pair
     r, dmax = 3, 20.0

@@ -1,7 +1,7 @@
-"""Pins of the oracle's precompute, cost (patch_eval_census,
-pmhstereo.cl:661-736) and KLT step (patch_census_klt2, :326-410): closed
-forms and special cases first, then a literal brute force on tiny images
-(tests/refimpl.py)."""
+"""Pins of the oracle's precompute, costs (patch_eval_census,
+pmhstereo.cl:661-736, and patch_eval_color_grad_weighted, :510-565) and KLT
+step (patch_census_klt2, :326-410): closed forms and special cases first,
+then a literal brute force on tiny images (tests/refimpl.py)."""
 import numpy as np
 import pytest
 
@@ -185,6 +185,139 @@ def test_cost_brute_force(oracle, C):
         got = oracle.cost_at(tl, tr, r, x, y, s, mincost=mc)
         want = refimpl.cost(tl, tr, r, x, y, s, mincost=mc)
         assert abs(got - want) <= 1e-6 * max(1.0, abs(want)), (x, y, s, mc)
+
+
+# ------------------------------------------- colour + gradient cost
+CG = dict(cost_kind=1)
+
+
+def test_cg_identical_images_cost_zero(oracle):
+    rng = np.random.default_rng(30)
+    img = rand_img(rng, 12, 16, 4)
+    tex = oracle.prepare(img, 2)
+    for (x, y) in [(0, 0), (5, 6), (15, 11)]:
+        assert oracle.cost_at(tex, tex, 2, x, y, FRONTO, **CG) == 0.0
+
+
+def test_cg_integer_shift_cost_zero(oracle):
+    # right(x) = left(x + d) on [0, W - d); the gradients of both agree at
+    # x' = x - d when x' - 1 >= 0 and x' + 1 < W - d (central differences),
+    # so every tap of the pixels x in [d + r + 1, W - r - 1) matches
+    rng = np.random.default_rng(31)
+    H, W, d, r = 10, 30, 4, 2
+    left = rand_img(rng, H, W, 4)
+    right = rand_img(rng, H, W, 4)
+    right[:, :W - d] = left[:, d:]
+    tl, tr = oracle.prepare(left, r), oracle.prepare(right, r)
+    for x in range(d + r + 1, W - r - 1):
+        for y in range(0, H):
+            assert oracle.cost_at(tl, tr, r, x, y, shift_plane(d), **CG) == 0
+            assert oracle.cost_at(tl, tr, r, x, y, shift_plane(d + 1),
+                                  **CG) > 0.01
+
+
+def test_cg_uniform_images_colour_truncation(oracle):
+    # uniform images: unit weights, zero gradients in the interior, so the
+    # cost is 0.05 x the truncated colour sum
+    H, W, r = 9, 9, 2
+    left = np.zeros((H, W, 4), np.float32)
+    right = np.zeros((H, W, 4), np.float32)
+    left[..., :3] = [0.2, 0.5, 0.7]
+    right[..., :3] = [0.21, 0.52, 0.8]     # diffs 0.01, 0.02, 0.1 (> 0.05)
+    left[..., 3] = right[..., 3] = 1.0
+    tl, tr = oracle.prepare(left, r), oracle.prepare(right, r)
+    want = 0.05 * (0.01 + 0.02 + 0.05)
+    assert abs(oracle.cost_at(tl, tr, r, 4, 4, FRONTO, **CG) - want) < 1e-7
+
+
+def test_cg_gradient_term_and_truncation(oracle):
+    # L: horizontal gray ramp of slope s per px; R: constant far from L.
+    # Every colour difference truncates (0.15); the gradient differences
+    # are (s, 0, 2s, 0) * gsum, truncated at 0.05 each; the weights vary,
+    # but C is the same at every interior tap, so the mean is C.
+    H, W, r = 9, 15, 2
+    gsum = float(np.float32(0.2126) + np.float32(0.7152) + np.float32(0.0722))
+    for slope, trunc in ((0.01, False), (0.04, True)):
+        left = np.tile((0.5 + slope * np.arange(W)).astype(np.float32),
+                       (H, 1))
+        right = np.zeros((H, W), np.float32) - 1.0
+        tl, tr = oracle.prepare(left, r), oracle.prepare(right, r)
+        g = [slope * gsum, 0.0, 2 * slope * gsum, 0.0]
+        sg = sum(min(abs(v), 0.05) for v in g)
+        want = 0.05 * 0.15 + 0.95 * sg
+        got = oracle.cost_at(tl, tr, r, 7, 4, FRONTO, **CG)
+        assert abs(got - want) < 2e-6, (slope, got, want)
+        if trunc:
+            assert abs(sg - (min(0.04 * gsum, 0.05) + 0.05)) < 1e-9
+
+
+def test_cg_no_tap_in_range_is_nan(oracle):
+    rng = np.random.default_rng(32)
+    tex = oracle.prepare(rand_img(rng, 8, 8, 4), 1)
+    for s in ([1, 0, -20, 0], [1, 0, 100, 0], [np.nan, 0, 0, 0],
+              [1, np.inf, 0, 0]):
+        assert np.isnan(oracle.cost_at(tex, tex, 1, 3, 3,
+                                       np.array(s, np.float32), **CG))
+
+
+def test_cg_has_no_invalid_plane_test(oracle):
+    # a plane steep in y (normal z < 1e-4) that maps its own row into the
+    # image: the census cost rejects it (1e5), this cost is finite, and
+    # the mincost argument has no effect (no early exit)
+    rng = np.random.default_rng(33)
+    tex = oracle.prepare(rand_img(rng, 11, 11, 4), 2)
+    s = np.array([1.0, 2e5, -2e5 * 5, 0.0], np.float32)
+    assert oracle.cost_at(tex, tex, 2, 5, 5, s) == np.float32(1e5)
+    c = oracle.cost_at(tex, tex, 2, 5, 5, s, **CG)
+    assert c == 0.0      # only row 5 taps count, at x' = x
+    s = shift_plane(0.5)
+    full = oracle.cost_at(tex, tex, 2, 5, 5, s, **CG)
+    assert 0 < full and oracle.cost_at(tex, tex, 2, 5, 5, s, mincost=0.0,
+                                       **CG) == full
+
+
+def test_cg_weight_factor(oracle):
+    # cost_weight 0 gives unit weights: the plain mean of the tap costs;
+    # other factors weigh the taps differently
+    rng = np.random.default_rng(34)
+    H, W, r = 7, 10, 2
+    left, right = rand_img(rng, H, W, 4), rand_img(rng, H, W, 4)
+    tl, tr = oracle.prepare(left, r), oracle.prepare(right, r)
+    s = shift_plane(1.0)
+    taps = []
+    for dy in range(-r, r + 1):
+        for dx in range(-r, r + 1):
+            # cost of a single tap: the patch of radius 0 at that pixel
+            taps.append(oracle.cost_at(tl, tr, 0, 5 + dx, 3 + dy, s, **CG))
+    mean0 = oracle.cost_at(tl, tr, r, 5, 3, s, cost_kind=1, cost_weight=0.0)
+    assert abs(mean0 - np.mean(taps)) < 1e-6
+    c16 = oracle.cost_at(tl, tr, r, 5, 3, s, **CG)
+    c25 = oracle.cost_at(tl, tr, r, 5, 3, s, cost_kind=1, cost_weight=25.5)
+    assert c16 != mean0 and c25 != c16
+    assert abs(c25 - refimpl.cost_cg(tl, tr, r, 5, 3, s, kw=25.5)) < 1e-6
+
+
+@pytest.mark.parametrize("C", [1, 4])
+def test_cg_cost_brute_force(oracle, C):
+    rng = np.random.default_rng(40 + C)
+    H, W, r = 7, 10, 2
+    left, right = rand_img(rng, H, W, C), rand_img(rng, H, W, C)
+    tl, tr = oracle.prepare(left, r), oracle.prepare(right, r)
+    nan = 0
+    for _ in range(80):
+        x, y = int(rng.integers(0, W)), int(rng.integers(0, H))
+        d = rng.uniform(0, 12)
+        s = np.array([1 + rng.normal(0, 0.1), rng.normal(0, 0.1),
+                      -d + rng.normal(0, 0.5), 0], np.float32)
+        kw = float(rng.choice([16.0, 25.5]))
+        got = oracle.cost_at(tl, tr, r, x, y, s, cost_kind=1, cost_weight=kw)
+        want = refimpl.cost_cg(tl, tr, r, x, y, s, kw=kw)
+        if np.isnan(want):
+            nan += 1
+            assert np.isnan(got), (x, y, s)
+            continue
+        assert abs(got - want) <= 1e-6 * max(1.0, abs(want)), (x, y, s, kw)
+    assert nan > 0      # some planes map the whole patch out of the image
 
 
 # ---------------------------------------------------------------- KLT

@@ -25,6 +25,25 @@ import pytest
 F = np.float32
 PROP_DRAWS = 14      # 2 x 4 neighbour offsets + 3 random plane + 3 refine
 
+# The propagate seeds its stream with
+#   RNG_init(&rng, (coord.y * width + coord.x) + iter,
+#            ( ( ( 2 * PROPSIZE + 1 ) * ( 2 * PROPSIZE + 1 ) - 1 )
+#              + NUMRNDTRIES ) * 3 + 2 * NUMRNDSAMPLE)     (pmhstereo.cl:942)
+# and RNG_init(x, linpos, numdraws) is MWC64X_Skip(x, ( linpos ) * numdraws)
+# (RNG.cl:54-57).  The preprocessor pastes the argument text as it stands,
+# with no parentheses around it, and C's precedence then applies to the
+# pasted tokens.  PROPSIZE 0, NUMRNDTRIES 2, NUMRNDSAMPLE 4
+# (pmhstereo.cl:28,32,33).
+_NUMDRAWS_TEXT = ("( ( ( 2 * {P} + 1 ) * ( 2 * {P} + 1 ) - 1 ) + {T} ) * 3"
+                  " + 2 * {S}").format(P=0, T=2, S=4)
+
+
+def prop_draw_start(p, it):
+    """Stream index of the first propagate draw: the macro's token text
+    evaluated with C's precedence for + and * (which Python shares)."""
+    text = "( {lin} ) * {nd}".format(lin=p + it, nd=_NUMDRAWS_TEXT)
+    return eval(text, {"__builtins__": {}})
+
 
 def bits(a):
     a = np.ascontiguousarray(a, np.float32).copy()
@@ -75,23 +94,25 @@ def view_sources(O, src, lr_src, y, x_target):
     return hits[:2]
 
 
-def propagate_py(O, t1, t2, R, dmax, lr, it, old, src_other):
+def propagate_py(O, t1, t2, R, dmax, lr, it, old, src_other,
+                 draw_start=prop_draw_start, cost=None):
+    cost = cost or {}
     H, W = old.shape[:2]
     out = np.zeros_like(old)
     theta = F(0.0)
     for y in range(H):
         for x in range(W):
             p = y * W + x
-            u = O.draws(PROP_DRAWS * (p + it), PROP_DRAWS)
+            u = O.draws(draw_start(p, it), PROP_DRAWS)
             self_ = old[y, x].astype(F).copy()
-            self_[3] = O.cost_at(t1, t2, R, x, y, self_, 1e10)
+            self_[3] = O.cost_at(t1, t2, R, x, y, self_, 1e10, **cost)
 
             def attempt(cand):
                 c = np.array(cand, F).copy()
                 ds = _sdist(O, self_, x, y, lr, dmax)
                 dc = _sdist(O, c, x, y, lr, dmax)
                 mincost = F(F(self_[3] + theta * ds) - theta * dc)
-                c[3] = O.cost_at(t1, t2, R, x, y, c, float(mincost))
+                c[3] = O.cost_at(t1, t2, R, x, y, c, float(mincost), **cost)
                 if F(c[3] + theta * dc) <= F(self_[3] + theta * ds):
                     return c
                 return self_
@@ -130,28 +151,34 @@ def propagate_py(O, t1, t2, R, dmax, lr, it, old, src_other):
     return out
 
 
+COSTS = {"census": {}, "color_grad": {"cost_kind": 1, "cost_weight": 25.5}}
+
+
+@pytest.mark.parametrize("kind", sorted(COSTS))
 @pytest.mark.parametrize("channels", [1, 4])
-def test_propagate_matches_sequential_statement(oracle, synth, channels):
+def test_propagate_matches_sequential_statement(oracle, synth, channels,
+                                                kind):
     O = oracle
+    cost = COSTS[kind]
     R, dmax = 2, 5.0
     L, Rt, _ = small_pair(synth, channels=channels, seed=7 + channels)
     tl, tr = O.prepare(L, R), O.prepare(Rt, R)
-    sl = O.init(tl, tr, R, dmax, 1)
-    sr = O.init(tr, tl, R, dmax, 0)
+    sl = O.init(tl, tr, R, dmax, 1, **cost)
+    sr = O.init(tr, tl, R, dmax, 0, **cost)
     H, W = sl.shape[:2]
     # iteration 0, left: empty view buffer
     nl, vbl = O.propagate(tl, tr, R, dmax, 1, 0, sl, O.viewbuf_empty(W, H),
-                          sr)
-    pl = propagate_py(O, tl, tr, R, dmax, 1, 0, sl, None)
+                          sr, **cost)
+    pl = propagate_py(O, tl, tr, R, dmax, 1, 0, sl, None, cost=cost)
     np.testing.assert_array_equal(bits(nl), bits(pl))
     # iteration 0, right: reads the left view's scatter
-    nr, _ = O.propagate(tr, tl, R, dmax, 0, 0, sr, vbl, nl)
-    pr = propagate_py(O, tr, tl, R, dmax, 0, 0, sr, nl)
+    nr, _ = O.propagate(tr, tl, R, dmax, 0, 0, sr, vbl, nl, **cost)
+    pr = propagate_py(O, tr, tl, R, dmax, 0, 0, sr, nl, cost=cost)
     np.testing.assert_array_equal(bits(nr), bits(pr))
     # iteration 1, left: reads the right view's scatter
     vbr = O.scatter(nr, 0)
-    nl2, _ = O.propagate(tl, tr, R, dmax, 1, 1, nl, vbr, nr)
-    pl2 = propagate_py(O, tl, tr, R, dmax, 1, 1, nl, nr)
+    nl2, _ = O.propagate(tl, tr, R, dmax, 1, 1, nl, vbr, nr, **cost)
+    pl2 = propagate_py(O, tl, tr, R, dmax, 1, 1, nl, nr, cost=cost)
     np.testing.assert_array_equal(bits(nl2), bits(pl2))
 
 
@@ -166,6 +193,37 @@ def test_statement_detects_a_wrong_draw_offset(oracle, synth):
     H, W = sl.shape[:2]
     nl, _ = O.propagate(tl, tr, R, dmax, 1, 0, sl, O.viewbuf_empty(W, H), sl)
     wrong = propagate_py(O, tl, tr, R, dmax, 1, 1, sl, None)
+    assert (bits(nl) != bits(wrong)).any()
+
+
+def test_prop_draw_start_is_the_macro_expansion(oracle):
+    """The oracle's first propagate draw is the reference macro's value:
+    8, 14, 20, 26 for linpos = p + iter = 0..3 (the preprocessed text), and
+    the token-text evaluation over a grid of pixels and iterations."""
+    O = oracle
+    assert [prop_draw_start(lin, 0) for lin in range(4)] == [8, 14, 20, 26]
+    assert [O.prop_draw_start(lin, 0) for lin in range(4)] == [8, 14, 20, 26]
+    for p in (0, 1, 2, 7, 63, 64, 1279, 921599, 2073599):
+        for it in (0, 1, 2, 7, 15):
+            assert O.prop_draw_start(p, it) == prop_draw_start(p, it)
+            assert O.prop_draw_start(p, it) == prop_draw_start(p + it, 0)
+    # the parenthesised reading, 14 (p + iter), agrees only at p + iter = 1
+    wrong = [lin for lin in range(64) if 14 * lin == prop_draw_start(lin, 0)]
+    assert wrong == [1]
+
+
+def test_statement_detects_the_parenthesised_draw_start(oracle, synth):
+    """A statement that seeds pixel p at 14 (p + iter), the reading with the
+    argument parenthesised, disagrees with the oracle's planes."""
+    O = oracle
+    R, dmax = 2, 5.0
+    L, Rt, _ = small_pair(synth)
+    tl, tr = O.prepare(L, R), O.prepare(Rt, R)
+    sl = O.init(tl, tr, R, dmax, 1)
+    H, W = sl.shape[:2]
+    nl, _ = O.propagate(tl, tr, R, dmax, 1, 0, sl, O.viewbuf_empty(W, H), sl)
+    wrong = propagate_py(O, tl, tr, R, dmax, 1, 0, sl, None,
+                         draw_start=lambda p, it: 14 * (p + it))
     assert (bits(nl) != bits(wrong)).any()
 
 
@@ -229,6 +287,34 @@ def test_propagate_invariants_plain(oracle, synth):
         np.testing.assert_array_equal(fresh[m], nl[..., 3][m])
         nr, vb2 = O.propagate(tr, tl, R, dmax, 0, it, sr, vb1, nl)
         assert (nr[..., 3] <= O.cost_field(tr, tl, R, sr)).all()
+        sl, sr = nl, nr
+
+
+def test_propagate_invariants_color_grad(oracle, synth):
+    """theta = 0 with the colour+gradient cost: a pixel's cost never
+    increases, a NaN cost (no tap in range) keeps its plane (the accept
+    rule's <= is false against NaN), and every stored cost equals a fresh
+    evaluation of the stored plane (no early exit)."""
+    O = oracle
+    cost = COSTS["color_grad"]
+    R, dmax = 3, 24.0
+    L, Rt, _ = synth.make_pair(48, 32, cells=4, dmax=dmax, channels=4,
+                               seed=12)
+    tl, tr = O.prepare(L, R), O.prepare(Rt, R)
+    sl = O.init(tl, tr, R, dmax, 1, **cost)
+    sr = O.init(tr, tl, R, dmax, 0, **cost)
+    H, W = sl.shape[:2]
+    assert np.isnan(sl[..., 3]).any()     # some init planes miss the image
+    vb2 = O.viewbuf_empty(W, H)
+    for it in range(3):
+        before = O.cost_field(tl, tr, R, sl, **cost)
+        nl, vb1 = O.propagate(tl, tr, R, dmax, 1, it, sl, vb2, sr, **cost)
+        stuck = np.isnan(before)
+        np.testing.assert_array_equal(bits(nl[stuck]), bits(sl[stuck]))
+        assert (nl[..., 3][~stuck] <= before[~stuck]).all()
+        fresh = O.cost_field(tl, tr, R, nl, **cost)
+        np.testing.assert_array_equal(bits(fresh), bits(nl[..., 3]))
+        nr, vb2 = O.propagate(tr, tl, R, dmax, 0, it, sr, vb1, nl, **cost)
         sl, sr = nl, nr
 
 

@@ -132,3 +132,48 @@ def test_to_ref_normal_depth(oracle):
     bad = np.array([np.nan, 0, 0, 0], np.float32)
     assert np.array_equal(oracle.to_ref_normal_depth(bad, 1, 1, 1),
                           np.array([0, 0, 1, 0], np.float32))
+
+
+def test_right_view_plane_from_the_mirrored_pass(oracle):
+    """The proposal's right-view planes: a network run on the mirrored pair
+    (both images flipped, swapped) sees the right view as a left view and
+    predicts, at column W - 1 - x, its disparity d' and normal n' in the
+    left view's convention (disparity slopes -n'x/n'z along its x, -n'y/n'z
+    along y).  The engine builds the right plane as
+    from_disparity_normal(x, y, -d', (n'x, -n'y, n'z)), the
+    pmhstereo_init_disparity_normal form for the right view
+    (pmhstereo.cl:818-819).  For a slanted surface seen from both views
+    that plane must map every right pixel to its left match."""
+    O = oracle
+    W = 200
+    d0, gx, gy, x0, y0 = 30.0, 0.08, -0.05, 90.0, 40.0
+
+    def d_left(x, y):
+        return d0 + gx * (x - x0) + gy * (y - y0)
+
+    def d_right(xr, y):     # the same surface: x_l = x_r + d_R, d_L(x_l) = d_R
+        return (d0 + gx * (xr - x0) + gy * (y - y0)) / (1.0 - gx)
+
+    # the left view's own plane from its normal: slopes -nx/nz = gx, ...
+    nl = np.array([-gx, -gy, 1.0], np.float64)
+    nl /= np.linalg.norm(nl)
+    sl = O.from_disparity_normal(x0, y0, d_left(x0, y0), nl.astype(np.float32))
+    for (x, y) in [(x0, y0), (120.0, 10.0), (60.0, 70.0)]:
+        assert abs((x - O.transform(sl, x, y)) - d_left(x, y)) < 1e-3
+    # the mirrored pass at column xm = W - 1 - xr: d'(xm, y) = d_R(xr, y),
+    # so d' slopes -gx/(1-gx) along xm and gy/(1-gx) along y
+    xr, y = 70.0, 30.0
+    sxm, sy = -gx / (1.0 - gx), gy / (1.0 - gx)
+    nm = np.array([-sxm, -sy, 1.0], np.float64)   # -n'x/n'z = sxm, ...
+    nm /= np.linalg.norm(nm)
+    nr = np.array([nm[0], -nm[1], nm[2]], np.float32)
+    sr = O.from_disparity_normal(xr, y, -d_right(xr, y), nr)
+    for (x2, y2) in [(xr, y), (xr + 25.0, y - 12.0), (xr - 30.0, y + 20.0)]:
+        xl = O.transform(sr, x2, y2)
+        assert abs((xl - x2) - d_right(x2, y2)) < 1e-3, (x2, y2)
+        # and the left view sees the same disparity at the match
+        assert abs(d_left(xl, y2) - (xl - x2)) < 1e-3
+    # the unflipped normal gives the wrong slope
+    bad = O.from_disparity_normal(xr, y, -d_right(xr, y), nm.astype(np.float32))
+    assert abs((O.transform(bad, xr, y + 20.0) - xr) -
+               d_right(xr, y + 20.0)) > 0.5
