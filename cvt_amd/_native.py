@@ -82,6 +82,7 @@ SIGNATURES = {
     "pm_pfm_read": (_i, [ctypes.c_char_p, _p, _i, _i, _i, _i]),
     "pm_pfm_write": (_i, [ctypes.c_char_p, _p, _i, _i, _i, _i]),
     "pm_eval_error": (_i, [_p, _i, _p, _i, _i, _i, _f, _f, _p, _p, _p]),
+    "pm_eval_error_device": (_i, [_p, _i, _p, _i, _i, _i, _f, _f, _p, _p]),
     "pm_error_histogram": (_i, [_p, _i, _p, _i, _i, _i, _i, _p, _i, _p]),
     "pm_default_histogram": (_i, [_i, _p, _p]),
 }

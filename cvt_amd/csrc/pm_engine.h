@@ -166,6 +166,13 @@ hipError_t launch_fill_state(hipStream_t st, const Geometry& g,
 
 hipError_t launch_fill4(hipStream_t st, size_t n, float4 v, float4* out);
 
+// Disparity::evaluateError on the device: counts[0..2] = error, valid,
+// all (zeroed first by a kernel, so the pair can be recorded in a graph)
+hipError_t launch_eval_error(hipStream_t st, const float* disp, int sd,
+                             const float* gt, int sg, int W, int H,
+                             float abs_thres, float rel_thres,
+                             unsigned long long* counts);
+
 hipError_t launch_draws(hipStream_t st, uint64_t k, int n, float* out);
 
 hipError_t launch_math(hipStream_t st, const float* in, int n, float* out);

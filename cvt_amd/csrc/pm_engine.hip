@@ -988,6 +988,42 @@ k_fill_normalmap(int W, int H, const float4* __restrict__ states,
   normals[p] = mk4(n.x, n.y, n.z, 0.0f);
 }
 
+// Disparity::evaluateError (cvt/vision/Disparity.cpp:201-233) on the
+// device: over the pixels with gt > 0, counts[2] += 1, counts[1] += 1 when
+// the estimate is > 0, counts[0] += 1 when |gt - est| > abs_thres and
+// |gt - est| / |gt| > rel_thres.  A block sums its pixels' counts, one
+// atomic per counter.
+__global__ void __launch_bounds__(256)
+k_eval_error(const float* __restrict__ disp, int sd, const float* __restrict__ gt,
+             int sg, int W, int H, float abs_thres, float rel_thres,
+             unsigned long long* __restrict__ counts) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  unsigned e = 0, v = 0, a = 0;
+  if (x < W && y < H) {
+    const float g = gt[(size_t)y * sg + x];
+    if (g > 0.0f) {
+      const float d = disp[(size_t)y * sd + x];
+      const float diff = fabsf(g - d);
+      e = (diff > abs_thres && diff / fabsf(g) > rel_thres) ? 1u : 0u;
+      v = d > 0.0f ? 1u : 0u;
+      a = 1u;
+    }
+  }
+  // wave sums by ballot, then one atomic per wave and counter
+  const unsigned long long be = __ballot(e), bv = __ballot(v),
+                           ba = __ballot(a);
+  if ((threadIdx.x & 63) == 0) {
+    if (be) atomicAdd(&counts[0], (unsigned long long)__popcll(be));
+    if (bv) atomicAdd(&counts[1], (unsigned long long)__popcll(bv));
+    if (ba) atomicAdd(&counts[2], (unsigned long long)__popcll(ba));
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 // draws [k + 16 t, k + 16 t + 16) by one jump and sixteen steps
 __global__ void __launch_bounds__(256)
 k_draws(uint64_t k, int n, float* __restrict__ out) {
@@ -1197,6 +1233,19 @@ hipError_t launch_fill_normalmap(hipStream_t st, const Geometry& g,
   PM_LAUNCH(k_fill_normalmap,
             dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY), kBlock2D, 0,
             st, g.W, g.H, states, normals);
+}
+
+hipError_t launch_eval_error(hipStream_t st, const float* disp, int sd,
+                             const float* gt, int sg, int W, int H,
+                             float abs_thres, float rel_thres,
+                             unsigned long long* counts) {
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counts, 3);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  PM_LAUNCH(k_eval_error, dim3((W + kBX - 1) / kBX, (H + kBY - 1) / kBY),
+            kBlock2D, 0, st, disp, sd, gt, sg, W, H, abs_thres, rel_thres,
+            counts);
 }
 
 hipError_t launch_draws(hipStream_t st, uint64_t k, int n, float* out) {

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "pm_engine.h"
 #include "pm_io.h"
 
 namespace {
@@ -224,6 +225,23 @@ int pm_eval_error(const float* disparity, int stride_d, const float* gt,
   *error = e;
   *valid = v;
   *all = a;
+  return PM_OK;
+}
+
+int pm_eval_error_device(const float* d_disparity, int stride_d,
+                         const float* d_gt, int stride_gt, int W, int H,
+                         float abs_thres, float rel_thres, uint64_t* d_counts,
+                         void* stream) {
+  if (!d_disparity || !d_gt || !d_counts)
+    return io_fail(PM_ERR_ARG, "NULL argument");
+  if (W < 1 || H < 1 || stride_d < W || stride_gt < W)
+    return io_fail(PM_ERR_ARG, "invalid size or stride");
+  const hipError_t e = pm::launch_eval_error(
+      static_cast<hipStream_t>(stream), d_disparity, stride_d, d_gt,
+      stride_gt, W, H, abs_thres, rel_thres,
+      reinterpret_cast<unsigned long long*>(d_counts));
+  if (e != hipSuccess)
+    return io_fail(PM_ERR_HIP, "evaluate error: %s", hipGetErrorString(e));
   return PM_OK;
 }
 

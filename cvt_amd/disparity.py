@@ -41,6 +41,18 @@ def evaluate_error(disparity, gt, abs_thres=1.0, rel_thres=0.0):
     return e.value, v.value, a.value
 
 
+def evaluate_error_device(d_disparity, d_gt, W, H, d_counts, abs_thres=1.0,
+                          rel_thres=0.0, stride_d=None, stride_gt=None,
+                          stream=None):
+    """Disparity::evaluateError on the device (pm_eval_error_device):
+    device pointers of W x H float maps; writes (error, valid, all) as
+    three uint64 at d_counts in stream order."""
+    _check(_native.lib().pm_eval_error_device(
+        ctypes.c_void_p(d_disparity), stride_d or W, ctypes.c_void_p(d_gt),
+        stride_gt or W, W, H, float(abs_thres), float(rel_thres),
+        ctypes.c_void_p(d_counts), stream))
+
+
 def evaluate_error_rate(disparity, gt, abs_thres=1.0, rel_thres=0.0):
     """Disparity::evaluateError's (error, density) form (Disparity.h:49-56):
     error / all and valid / all."""

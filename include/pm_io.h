@@ -61,6 +61,16 @@ int pm_eval_error(const float* disparity, int stride_d, const float* gt,
                   float rel_thres, uint64_t* error, uint64_t* valid,
                   uint64_t* all);
 
+/* pm_eval_error on the device: d_disparity, d_gt in HIP device memory of
+ * the current device, d_counts = 3 uint64 (error, valid, all) written in
+ * stream order on `stream` (a hipStream_t, NULL = the null stream): two
+ * kernels, no host synchronisation and no memset, so the call can be
+ * recorded by stream capture. */
+int pm_eval_error_device(const float* d_disparity, int stride_d,
+                         const float* d_gt, int stride_gt, int W, int H,
+                         float abs_thres, float rel_thres,
+                         uint64_t* d_counts, void* stream);
+
 /* Disparity::histogramErrorAbsolute / histogramErrorRelative
  * (Disparity.cpp:236-318): over pixels where both maps are finite and
  * > 0, the fraction whose error e = |test - gt| (relative: divided by

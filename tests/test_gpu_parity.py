@@ -639,3 +639,32 @@ def test_random_configurations_bitexact(oracle, synth, W, H, C, r, dmax, cw,
     assert same(df, want_df), first_diff(df, want_df)
     want_n = oracle.fill_normalmap(chk)
     assert same(nrm, want_n), first_diff(nrm, want_n)
+
+
+def test_eval_error_device_matches_host(synth):
+    """pm_eval_error_device against the host pm_eval_error
+    (Disparity::evaluateError): the same integer counts, with NaN
+    estimates, rejected pixels (0), pixels without ground truth (gt <= 0)
+    and a relative threshold, on a strided estimate."""
+    import torch
+    rng = np.random.default_rng(41)
+    H, W = 203, 317
+    gt = rng.uniform(-2, 60, (H, W)).astype(np.float32)
+    est = gt + rng.normal(0, 1.5, (H, W)).astype(np.float32)
+    est[::7, ::5] = 0.0
+    est[3::11, 2::13] = np.nan
+    pad = np.zeros((H, W + 9), np.float32)
+    pad[:, :W] = est
+    d_est = torch.from_numpy(pad).cuda()
+    d_gt = torch.from_numpy(gt).cuda()
+    counts = torch.zeros(3, dtype=torch.int64, device="cuda")
+    for abs_t, rel_t in ((1.0, 0.0), (2.0, 0.05), (0.5, 0.0)):
+        counts.fill_(123)           # the call zeroes before counting
+        torch.cuda.synchronize()
+        cvt_amd.disparity.evaluate_error_device(
+            d_est.data_ptr(), d_gt.data_ptr(), W, H, counts.data_ptr(),
+            abs_t, rel_t, stride_d=W + 9)
+        torch.cuda.synchronize()
+        got = tuple(int(v) for v in counts.cpu())
+        want = cvt_amd.disparity.evaluate_error(est, gt, abs_t, rel_t)
+        assert got == want, (abs_t, rel_t, got, want)
