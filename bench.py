@@ -110,6 +110,49 @@ def host_cpu():
     return {"cpu_model": model, "nproc": nproc, "threads": O.num_threads()}
 
 
+def schedules_by_iteration(dL, dR, gt, cost_kind, cost_weight, stream):
+    """bad-1.0px after each of the 8 iterations, and the time of 8
+    iterations as one pm_iterate call (HIP events on the engine's stream),
+    for the reference's Jacobi sweep and the red-black schedule."""
+    import numpy as np
+    import torch
+
+    import cvt_amd
+    out = {}
+    dD = torch.empty((H, W), dtype=torch.float32, device=dL.device)
+    for name, sched in (("jacobi", cvt_amd.PM_SCHEDULE_JACOBI),
+                        ("red_black", cvt_amd.PM_SCHEDULE_RED_BLACK)):
+        with cvt_amd.Context(radius=RADIUS, disparity_max=DMAX,
+                             stream=stream.cuda_stream, cost_kind=cost_kind,
+                             cost_weight=cost_weight,
+                             schedule=sched) as ctx, \
+                torch.cuda.stream(stream):
+            bad = []
+            ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
+            ctx.setup()
+            for it in range(ITERS):
+                ctx.iterate(1)
+                ctx.disparity_device(dD.data_ptr(),
+                                     cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP,
+                                     MAXDEG)
+                torch.cuda.synchronize()
+                bad.append(cvt_amd.disparity.bad_pixel_rate(
+                    dD.cpu().numpy(), gt, 1.0))
+            ms = []
+            for _ in range(3):
+                ctx.setup()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                ctx.iterate(ITERS)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms.append(e0.elapsed_time(e1))
+        out[name] = {"bad_1px_by_iteration": bad,
+                     "ms_per_8_iterations": float(np.min(ms))}
+    return out
+
+
 def cpu_baseline(L, R, gt, cost, gpu_disp):
     """The oracle (C, OpenMP on all host cores): one full run of the 720p
     pair (setup, all 8 iterations, LR check and disparity, the GPU step's
@@ -235,6 +278,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-schedules", action="store_true",
+                    help="skip the bad-1px-by-iteration comparison of the "
+                         "Jacobi and red-black schedules")
     ap.add_argument("--pipeline", choices=["plain", "huber"], default="plain",
                     help="plain PatchMatch (the headline workload) or the "
                          "PM-Huber pipeline (theta schedule plus "
@@ -421,6 +467,10 @@ def main():
             "avg_launch_ms": rof_ms,
             "steps_per_launch": ROF_STEPS_PER_LAUNCH,
             "iteration_ms": float(np.mean(iter_ms))}
+    if not huber and not args.no_schedules:
+        log("bench: bad-1px by iteration, Jacobi and red-black schedules")
+        result["schedules"] = schedules_by_iteration(dL, dR, gt, cost_kind,
+                                                     cost_weight, stream)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) on a bounded sample")
         result["cpu_baseline"] = (cpu_baseline_huber(L, R) if huber

@@ -14,6 +14,7 @@ PM_OK, PM_ERR_ARG, PM_ERR_STATE, PM_ERR_HIP, PM_ERR_NOMEM, PM_ERR_IO = range(6)
 PM_PP_NONE, PM_PP_LRCHECK, PM_PP_LRCHECK_FILL = range(3)
 PM_SMOOTH_NONE, PM_SMOOTH_HUBER, PM_SMOOTH_DEPTHMAP = range(3)
 PM_COST_CENSUS, PM_COST_COLOR_GRAD = range(2)
+PM_SCHEDULE_JACOBI, PM_SCHEDULE_RED_BLACK = range(2)
 
 _p = ctypes.c_void_p
 _i = ctypes.c_int
@@ -27,6 +28,7 @@ class PmParams(ctypes.Structure):
                 ("color_weight", ctypes.c_float),
                 ("cost_kind", ctypes.c_int),
                 ("cost_weight", ctypes.c_float),
+                ("schedule", ctypes.c_int),
                 ("smoothing", ctypes.c_int),
                 ("theta_min", ctypes.c_float),
                 ("schedule_iterations", ctypes.c_int),

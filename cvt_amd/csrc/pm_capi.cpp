@@ -159,6 +159,9 @@ int check_params(pm_ctx* c, const pm_params* p) {
     return fail(c, PM_ERR_ARG, "color_weight must be finite and >= 0");
   if (p->cost_kind != PM_COST_CENSUS && p->cost_kind != PM_COST_COLOR_GRAD)
     return fail(c, PM_ERR_ARG, "unknown cost_kind %d", p->cost_kind);
+  if (p->schedule != PM_SCHEDULE_JACOBI &&
+      p->schedule != PM_SCHEDULE_RED_BLACK)
+    return fail(c, PM_ERR_ARG, "unknown schedule %d", p->schedule);
   if (!(p->cost_weight >= 0.0f) || !std::isfinite(p->cost_weight))
     return fail(c, PM_ERR_ARG, "cost_weight must be finite and >= 0");
   if (p->smoothing != PM_SMOOTH_NONE && p->smoothing != PM_SMOOTH_HUBER &&
@@ -413,8 +416,12 @@ int block_rows(const pm_ctx* c) {
   return (c->geo.H + pm::kBlockRows - 1) / pm::kBlockRows;
 }
 
-// true when every band of the split schedule gets a block row
-bool can_split(const pm_ctx* c) { return block_rows(c) >= kSplitParts; }
+// true when every band of the split schedule gets a block row; the
+// red-black schedule runs its two passes whole, on the context's stream
+bool can_split(const pm_ctx* c) {
+  return block_rows(c) >= kSplitParts &&
+         c->prm.schedule == PM_SCHEDULE_JACOBI;
+}
 
 // first block row of band k (band k ends where band k + 1 starts)
 int band_first(const pm_ctx* c, int k) {
@@ -494,11 +501,25 @@ int propagate_split(pm_ctx* c, const pm::PropagateArgs& a, int view,
 
 // One view's propagate: the whole image on the context's stream, or as
 // two halves when the caller set up the split schedule (split_rebase).
+// The red-black schedule runs it as two passes over the pixels of either
+// colour, the second reading the first's new planes for its colour-0
+// neighbours.
 int propagate_view(pm_ctx* c, const pm::PropagateArgs& a, int view,
                    unsigned long long* vb, unsigned long long* counters,
                    bool split) {
   if (split) return propagate_split(c, a, view, vb, counters);
   PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vb));
+  if (c->prm.schedule == PM_SCHEDULE_RED_BLACK) {
+    pm::Geometry g = c->geo;
+    pm::PropagateArgs b = a;
+    b.fresh = a.out;
+    for (int color = 0; color < 2; ++color) {
+      g.color = color;
+      PM_HIP(c, pm::launch_propagate(c->stream, g, b, counters,
+                                     c->prop_lds_floor));
+    }
+    return PM_OK;
+  }
   PM_HIP(c, pm::launch_propagate(c->stream, c->geo, a, counters,
                                  c->prop_lds_floor));
   return PM_OK;
@@ -720,6 +741,7 @@ void pm_params_default(pm_params* p) {
   p->color_weight = 12.0f;
   p->cost_kind = PM_COST_CENSUS;
   p->cost_weight = 16.0f;
+  p->schedule = PM_SCHEDULE_JACOBI;
   p->smoothing = PM_SMOOTH_NONE;
   p->theta_min = 1e-4f;
   p->schedule_iterations = 0;

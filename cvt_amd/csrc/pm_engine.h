@@ -56,6 +56,9 @@ struct Geometry {
   // support-weight factor of the colour+gradient cost (16 in the
   // reference, pmhstereo.cl:531)
   float cost_weight = 16.0f;
+  // red-black propagate pass: -1 = every pixel (the reference's Jacobi
+  // sweep), 0 / 1 = only the pixels with (x + y) & 1 equal to it
+  int color = -1;
 };
 
 hipError_t launch_prepare(hipStream_t st, const Geometry& g, const float* img,
@@ -97,6 +100,9 @@ struct PropagateArgs {
   float4* out = nullptr;
   float dmax = 0.0f, theta = 0.0f;
   int lr = 1, iter = 0;
+  // red-black pass 1 (Geometry::color == 1): the pass-0 pixels' new planes,
+  // read instead of `old` for neighbours of colour 0 (may alias `out`)
+  const float4* fresh = nullptr;
 };
 
 // counters (optional, 2 x u64): cost taps and KLT taps evaluated, counted

@@ -72,17 +72,26 @@ constexpr int kTX = kBlockCols;  // block width = one wavefront
 constexpr int kTY = kBlockRows;  // wavefronts (rows) per block
 constexpr int kXcds = 8;
 
+// image columns a block covers: one per lane, or every other column in a
+// red-black pass (the lanes take the pass's colour only)
+__host__ __device__ __forceinline__ int block_cols(int color) {
+  return color >= 0 ? 2 * kTX : kTX;
+}
+
 // the blocks of a launch: every block column, the launch's block rows
 __host__ __forceinline__ dim3 grid_for(const Geometry& g) {
   const int nby = g.nby > 0 ? g.nby : (g.H + kTY - 1) / kTY;
-  return dim3((unsigned)((g.W + kTX - 1) / kTX), (unsigned)nby);
+  const int cols = block_cols(g.color);
+  return dim3((unsigned)((g.W + cols - 1) / cols), (unsigned)nby);
 }
 
 // LDS bytes of the block kernels: the own-view tile (colour and sign
 // bits), followed for the colour+gradient cost by a tile of the own view's
 // gradients
-__host__ __forceinline__ size_t block_lds_bytes(int R, int kind) {
-  const size_t tile = (size_t)(kTX + 2 * R) * (kTY + 2 * R) * sizeof(float4);
+__host__ __forceinline__ size_t block_lds_bytes(int R, int kind,
+                                                int color = -1) {
+  const size_t tile = (size_t)(block_cols(color) + 2 * R) * (kTY + 2 * R) *
+                      sizeof(float4);
   return kind == kCostColorGrad ? 2 * tile : tile;
 }
 
@@ -131,17 +140,20 @@ struct Lane {
 
 // Block prologue shared by the cost-evaluating kernels: stages the own-view
 // tile (r, g, b, gradient sign bits), and for kGrad the tile of its
-// gradients after it, and returns the lane context.  Must be reached by
-// every thread of the block.
-template <bool kGrad>
+// gradients after it, and returns the lane context.  For kRB (a red-black
+// pass, g.color = 0 or 1) the block covers 2 x 64 columns and lane i takes
+// column 2 i + ((y + color) & 1) of its row, the pixel of the pass's
+// colour.  Must be reached by every thread of the block.
+template <bool kGrad, bool kRB = false>
 __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
                             const ViewBufs& other, float4* smem) {
   int bx, by;
   xcd_block(&bx, &by);
   by += g.by0;
   const int R = g.R, W = g.W, H = g.H;
-  const int TW = kTX + 2 * R, TH = kTY + 2 * R;
-  const int x0 = bx * kTX, y0 = by * kTY;
+  const int cols = kRB ? 2 * kTX : kTX;
+  const int TW = cols + 2 * R, TH = kTY + 2 * R;
+  const int x0 = bx * cols, y0 = by * kTY;
   const int tid = threadIdx.y * kTX + threadIdx.x;
   const size_t gplane = (size_t)g.TS * H;
   for (int i = tid; i < TW * TH; i += kTX * kTY) {
@@ -160,13 +172,15 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
   __syncthreads();
   // the row index is uniform per wavefront: keep it scalar
   const int ly = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const int lx = kRB ? 2 * (int)threadIdx.x + ((y0 + ly + g.color) & 1)
+                     : (int)threadIdx.x;
   Lane L;
   L.W = W;
   L.H = H;
   L.R = R;
   L.Wf = (float)W;
   L.cw = g.color_weight;
-  L.x = x0 + (int)threadIdx.x;
+  L.x = x0 + lx;
   L.y = y0 + ly;
   L.valid = L.x < W && L.y < H;
   L.xf = (float)L.x;
@@ -177,7 +191,7 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
   L.kw = g.cost_weight;
   L.TS = g.TS;
   L.GP = (size_t)g.TS * g.H;
-  L.trow0 = smem + ly * TW + threadIdx.x;
+  L.trow0 = smem + ly * TW + lx;
   L.tex2 = other.tex + kTexPadL;
   L.tc = L.trow0[R * TW + R];
   L.bilw = L.valid
@@ -743,11 +757,14 @@ __device__ __forceinline__ void scatter(float4 self, int x, int y, int W,
 // to scratch.  Measured on the 720p pair against the unconstrained build
 // (107 VGPRs, four wavefronts): 8 iterations 52.1 -> 51.8 ms; six
 // wavefronts (80 VGPRs, more spills) gain nothing.
-template <int kKind, bool kCount>
+// kRB: one pass of the red-black schedule (g.color); pass 1 reads the
+// neighbours of colour 0 from a.fresh, the planes pass 0 just wrote.
+template <int kKind, bool kCount, bool kRB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
 k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
   extern __shared__ float4 smem[];
-  const Lane L = block_setup<kKind == kCostColorGrad>(g, a.self, a.other, smem);
+  const Lane L =
+      block_setup<kKind == kCostColorGrad, kRB>(g, a.self, a.other, smem);
   const int W = g.W, H = g.H;
   uint32_t ncost = 0;
   uint32_t* nc = kCount ? &ncost : nullptr;
@@ -787,7 +804,9 @@ k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
     float4 cand = mk4(0.0f, 0.0f, 0.0f, 0.0f);
     if (L.valid) {
       const int nx = clampi(L.x + ox, 0, W - 1), ny = clampi(L.y + oy, 0, H - 1);
-      cand = a.old[(size_t)ny * W + nx];
+      const float4* src = a.old;
+      if (kRB && g.color == 1 && ((nx + ny) & 1) == 0) src = a.fresh;
+      cand = src[(size_t)ny * W + nx];
     }
     try_candidate<kKind, kCount>(L, c, self, cand, true, nc);
   }
@@ -1154,31 +1173,42 @@ hipError_t launch_viewbuf_clear(hipStream_t st, const Geometry& g,
   PM_LAUNCH(k_viewbuf_clear, dim3(blocks), dim3(256), 0, st, g.npix, vbuf);
 }
 
-template <int kKind>
+template <int kKind, bool kRB>
 static hipError_t launch_propagate_kind(hipStream_t st, const Geometry& g,
                                         const PropagateArgs& a,
                                         unsigned long long* counters,
                                         size_t lds_floor) {
-  const size_t lds = std::max(block_lds_bytes(g.R, kKind), lds_floor);
+  const size_t lds =
+      std::max(block_lds_bytes(g.R, kKind, kRB ? g.color : -1), lds_floor);
   if (counters) {
-    const hipError_t e = allow_lds(k_propagate<kKind, true>, lds);
+    const hipError_t e = allow_lds(k_propagate<kKind, true, kRB>, lds);
     if (e != hipSuccess) return e;
-    PM_LAUNCH((k_propagate<kKind, true>), grid_for(g), kBlockTile, lds, st, g,
-              a, counters);
+    PM_LAUNCH((k_propagate<kKind, true, kRB>), grid_for(g), kBlockTile, lds,
+              st, g, a, counters);
   }
-  const hipError_t e = allow_lds(k_propagate<kKind, false>, lds);
+  const hipError_t e = allow_lds(k_propagate<kKind, false, kRB>, lds);
   if (e != hipSuccess) return e;
-  PM_LAUNCH((k_propagate<kKind, false>), grid_for(g), kBlockTile, lds, st, g,
-            a, counters);
+  PM_LAUNCH((k_propagate<kKind, false, kRB>), grid_for(g), kBlockTile, lds,
+            st, g, a, counters);
 }
 
 hipError_t launch_propagate(hipStream_t st, const Geometry& g,
                             const PropagateArgs& a,
                             unsigned long long* counters, size_t lds_floor) {
+  if (g.color != -1 && g.color != 0 && g.color != 1)
+    return hipErrorInvalidValue;
+  if (g.color >= 0 && g.color == 1 && !a.fresh) return hipErrorInvalidValue;
+  const bool rb = g.color >= 0;
   if (g.cost_kind == kCostColorGrad)
-    return launch_propagate_kind<kCostColorGrad>(st, g, a, counters,
-                                                 lds_floor);
-  return launch_propagate_kind<kCostCensus>(st, g, a, counters, lds_floor);
+    return rb ? launch_propagate_kind<kCostColorGrad, true>(st, g, a, counters,
+                                                           lds_floor)
+              : launch_propagate_kind<kCostColorGrad, false>(st, g, a,
+                                                            counters,
+                                                            lds_floor);
+  return rb ? launch_propagate_kind<kCostCensus, true>(st, g, a, counters,
+                                                      lds_floor)
+            : launch_propagate_kind<kCostCensus, false>(st, g, a, counters,
+                                                       lds_floor);
 }
 
 hipError_t launch_scatter(hipStream_t st, const Geometry& g,

@@ -17,6 +17,7 @@ import numpy as np
 from . import _native
 from ._native import (PM_COST_CENSUS, PM_COST_COLOR_GRAD,  # noqa
                       PM_PP_LRCHECK, PM_PP_LRCHECK_FILL, PM_PP_NONE,
+                      PM_SCHEDULE_JACOBI, PM_SCHEDULE_RED_BLACK,
                       PM_SMOOTH_DEPTHMAP, PM_SMOOTH_HUBER, PM_SMOOTH_NONE,
                       PmParams)
 
@@ -57,7 +58,8 @@ class Context:
     def __init__(self, radius=5, disparity_max=64.0, color_weight=12.0,
                  smoothing=PM_SMOOTH_NONE, theta_min=1e-4,
                  schedule_iterations=0, device=0, stream=None,
-                 cost_kind=PM_COST_CENSUS, cost_weight=16.0):
+                 cost_kind=PM_COST_CENSUS, cost_weight=16.0,
+                 schedule=PM_SCHEDULE_JACOBI):
         self._lib = _native.lib()
         p = PmParams()
         self._lib.pm_params_default(ctypes.byref(p))
@@ -66,6 +68,7 @@ class Context:
         p.color_weight = float(color_weight)
         p.cost_kind = int(cost_kind)
         p.cost_weight = float(cost_weight)
+        p.schedule = int(schedule)
         p.smoothing = int(smoothing)
         p.theta_min = float(theta_min)
         p.schedule_iterations = int(schedule_iterations)

@@ -85,6 +85,20 @@ typedef enum pm_cost_kind {
     PM_COST_COLOR_GRAD = 1
 } pm_cost_kind;
 
+/* Order in which a view's propagate visits its pixels */
+typedef enum pm_schedule {
+    /* the reference's Jacobi sweep: every pixel reads the previous field
+     * (pmhstereo_propagate_view reads `old`, writes `output`,
+     * pmhstereo.cl:880-1071) */
+    PM_SCHEDULE_JACOBI = 0,
+    /* not in the reference: two passes over the checkerboard colours
+     * ((x + y) even, then odd); the second pass's pixels see the first
+     * pass's new planes at their colour-0 random neighbours, so good planes
+     * spread twice per iteration.  Same candidates, draws, cost and accept
+     * rule; deterministic; runs without the split schedule */
+    PM_SCHEDULE_RED_BLACK = 1
+} pm_schedule;
+
 typedef struct pm_params {
     int radius;          /* patch half-width r: the reference's `patchsize`
                             (PMHuberStereo.h:47; loops -r..r,
@@ -98,6 +112,7 @@ typedef struct pm_params {
                             the reference (pmhstereo.cl:531); a gamma of
                             Bleyer's exp(-dI / gamma) on 0-255 data is
                             255 / gamma on [0, 1] data (gamma 10: 25.5) */
+    int schedule;        /* pm_schedule, default PM_SCHEDULE_JACOBI */
     int smoothing;       /* pm_smoothing, default PM_SMOOTH_NONE */
     float theta_min;     /* PM-Huber: first theta, 1e-4 as disparityMap
                             passes it (PMHuberStereo.cpp:92) */

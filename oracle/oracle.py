@@ -47,11 +47,11 @@ _SIGS = {
                               _p]),
     "pmo_viewbuf_clear": (None, [_p, _i, _i]),
     "pmo_propagate": (None, [_p, _p, _i, _i, _i, _f, _i, _f, _f, _f, _i, _i,
-                             _p, _p, _p, _p, _p, _p]),
+                             _p, _p, _p, _p, _p, _p, _i, _p]),
     "pmo_consistency": (None, [_p, _p, _i, _i, _f, _f, _p]),
     "pmo_to_disparity": (None, [_p, _i, _i, _f, _i, _p]),
-    "pmo_run": (_i, [_p, _p, _i, _i, _i, _i, _i, _f, _i, _i, _f, _f, _f, _p,
-                     _p, _p, _p]),
+    "pmo_run": (_i, [_p, _p, _i, _i, _i, _i, _i, _f, _i, _i, _f, _i, _f, _f,
+                     _p, _p, _p, _p]),
     "pmo_scatter": (None, [_p, _i, _i, _i, _p]),
     "pmo_fill_depthmap": (None, [_p, _i, _i, _f, _p]),
     "pmo_fill_normalmap": (None, [_p, _i, _i, _p]),
@@ -274,11 +274,18 @@ def viewbuf_empty(W, H):
 def propagate(tex1, tex2, radius, dmax, lr, it, old, viewin=None,
               src_other=None, smooth=None, theta=0.0, scatter=True,
               color_weight=COLOR_WEIGHT, cost_kind=COST_CENSUS,
-              cost_weight=CG_WEIGHT):
-    """One Jacobi propagate of one view; returns (new_states, viewout)."""
+              cost_weight=CG_WEIGHT, color=-1, fresh=None, out=None,
+              viewout=None):
+    """One Jacobi propagate of one view, or with color 0 / 1 one pass of
+    the red-black schedule (fresh: pass 0's output, read by pass 1);
+    returns (new_states, viewout).  out / viewout: existing arrays to
+    write into (a red-black pair of passes shares them)."""
     H, W = tex1.shape[:2]
-    out = np.zeros((H, W, 4), np.float32)
-    vout = viewbuf_empty(W, H) if scatter else None
+    if out is None:
+        out = np.zeros((H, W, 4), np.float32)
+    vout = viewout
+    if vout is None and scatter:
+        vout = viewbuf_empty(W, H)
     if viewin is not None:
         viewin = np.ascontiguousarray(viewin, np.uint64)
         src_other = _f32(src_other, (H, W, 4))
@@ -287,7 +294,10 @@ def propagate(tex1, tex2, radius, dmax, lr, it, old, viewin=None,
                         float(cost_weight), float(dmax), float(theta),
                         int(lr), int(it), _ptr(_f32(old, (H, W, 4))),
                         _ptr(_f32(smooth) if smooth is not None else None),
-                        _ptr(viewin), _ptr(src_other), _ptr(vout), _ptr(out))
+                        _ptr(viewin), _ptr(src_other), _ptr(vout), _ptr(out),
+                        int(color),
+                        _ptr(_f32(fresh, (H, W, 4)) if fresh is not None
+                             else None))
     return out, vout
 
 
@@ -307,8 +317,12 @@ def to_disparity(states, lr=1, scale=1.0):
     return out
 
 
+SCHEDULE_JACOBI, SCHEDULE_RED_BLACK = 0, 1
+
+
 def run(left, right, radius, dmax, iters, maxdisp=1.0, maxdeg=180.0,
-        trace=False, cost_kind=COST_CENSUS, cost_weight=CG_WEIGHT):
+        trace=False, cost_kind=COST_CENSUS, cost_weight=CG_WEIGHT,
+        schedule=SCHEDULE_JACOBI):
     """Plain PatchMatch pipeline; returns dict of stateL, stateR, disp,
     and (if trace) cost_trace [iters, 2, H, W]."""
     left, W, H, C, stride = _img_args(left)
@@ -320,7 +334,8 @@ def run(left, right, radius, dmax, iters, maxdisp=1.0, maxdeg=180.0,
     tr = np.zeros((iters, 2, H, W), np.float32) if trace else None
     rc = lib().pmo_run(_ptr(left), _ptr(right), W, H, C, stride, int(radius),
                        float(dmax), int(iters), int(cost_kind),
-                       float(cost_weight), float(maxdisp), float(maxdeg),
+                       float(cost_weight), int(schedule), float(maxdisp),
+                       float(maxdeg),
                        _ptr(sL), _ptr(sR), _ptr(disp), _ptr(tr))
     if rc != 0:
         raise MemoryError("pmo_run failed")

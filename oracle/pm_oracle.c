@@ -933,7 +933,8 @@ void pmo_propagate(const float* tex1, const float* tex2, int W, int H,
                    int radius, float cw, int kind, float kw, float dmax,
                    float theta, int lr, int iter, const float* old,
                    const float* smooth, const uint64_t* viewin,
-                   const float* src_other, uint64_t* viewout, float* out)
+                   const float* src_other, uint64_t* viewout, float* out,
+                   int color, const float* fresh)
 {
     const cost_model m = {cw, kind, kw};
     /* rows are independent: the scatter never leaves the row, so each
@@ -941,6 +942,7 @@ void pmo_propagate(const float* tex1, const float* tex2, int W, int H,
 #pragma omp parallel for schedule(dynamic, 1)
     for (int y = 0; y < H; y++) {
         for (int x = 0; x < W; x++) {
+            if (color >= 0 && ((x + y) & 1) != color) continue;
             size_t p = (size_t)y * W + x;
             prop_ctx c = {tex1, tex2, W, H, radius, &m, dmax, theta, lr,
                           (float)x, (float)y, mk4(0, 0, 0, 0), x, y};
@@ -970,7 +972,10 @@ void pmo_propagate(const float* tex1, const float* tex2, int W, int H,
                 int nx = x + ox, ny = y + oy;
                 nx = nx < 0 ? 0 : (nx > W - 1 ? W - 1 : nx);
                 ny = ny < 0 ? 0 : (ny > H - 1 ? H - 1 : ny);
-                try_candidate(&c, &self, load4(old, (size_t)ny * W + nx));
+                /* red-black pass 1: colour-0 neighbours from pass 0 */
+                const float* src =
+                    (color == 1 && ((nx + ny) & 1) == 0) ? fresh : old;
+                try_candidate(&c, &self, load4(src, (size_t)ny * W + nx));
             }
 
             /* random plane, pmhstereo.cl:996-1002 */
@@ -1450,10 +1455,30 @@ int pmo_num_threads(void)
 /* Plain PatchMatch: the disparityMapAll sequence (PMHuberStereo.cpp:
  * 406-581) with theta = 0, cleared smooth field and no PD smoothing,
  * then checkConsistencyLR + convertDisparity (PMHuberStereo.cpp:129-134). */
+/* one view's propagate in the run's schedule: the whole view (Jacobi) or
+ * the two red-black passes into the same output and scatter buffer */
+static void propagate_sched(const float* t1, const float* t2, int W, int H,
+                            int radius, float cw, int kind, float kw,
+                            float dmax, int lr, int it, const float* old,
+                            const uint64_t* viewin, const float* src_other,
+                            uint64_t* viewout, float* out, int schedule)
+{
+    if (schedule == PMO_SCHEDULE_RED_BLACK) {
+        for (int color = 0; color < 2; color++)
+            pmo_propagate(t1, t2, W, H, radius, cw, kind, kw, dmax, 0.0f, lr,
+                          it, old, NULL, viewin, src_other, viewout, out,
+                          color, out);
+        return;
+    }
+    pmo_propagate(t1, t2, W, H, radius, cw, kind, kw, dmax, 0.0f, lr, it, old,
+                  NULL, viewin, src_other, viewout, out, -1, NULL);
+}
+
 int pmo_run(const float* left, const float* right, int W, int H,
             int channels, int stride, int radius, float dmax, int iters,
-            int kind, float kw, float maxdispdiff, float maxdeg,
-            float* stateL, float* stateR, float* disp, float* cost_trace)
+            int kind, float kw, int schedule, float maxdispdiff,
+            float maxdeg, float* stateL, float* stateR, float* disp,
+            float* cost_trace)
 {
     const float cw = 12.0f;
     size_t n = (size_t)W * H;
@@ -1477,13 +1502,12 @@ int pmo_run(const float* left, const float* right, int W, int H,
     int cur = 0;
     for (int it = 0; it < iters; it++) {
         pmo_viewbuf_clear(vb1, W, H);
-        pmo_propagate(texL, texR, W, H, radius, cw, kind, kw,
-                      dmax, 0.0f, 1, it,
-                      sL[cur], NULL, vb2, sR[cur], vb1, sL[1 - cur]);
+        propagate_sched(texL, texR, W, H, radius, cw, kind, kw, dmax, 1, it,
+                        sL[cur], vb2, sR[cur], vb1, sL[1 - cur], schedule);
         pmo_viewbuf_clear(vb2, W, H);
-        pmo_propagate(texR, texL, W, H, radius, cw, kind, kw,
-                      dmax, 0.0f, 0, it,
-                      sR[cur], NULL, vb1, sL[1 - cur], vb2, sR[1 - cur]);
+        propagate_sched(texR, texL, W, H, radius, cw, kind, kw, dmax, 0, it,
+                        sR[cur], vb1, sL[1 - cur], vb2, sR[1 - cur],
+                        schedule);
         cur = 1 - cur;
         if (cost_trace) {
             for (size_t i = 0; i < n; i++) {
@@ -1555,11 +1579,11 @@ int pmo_run_huber(const float* left, const float* right, int W, int H,
         pmo_viewbuf_clear(vb1, W, H);
         pmo_propagate(texL, texR, W, H, radius, cw, PMO_COST_CENSUS,
                       16.0f, dmax, theta * 50.0f, 1,
-                      it, sL[cur], smL, vb2, sR[cur], vb1, sL[1 - cur]);
+                      it, sL[cur], smL, vb2, sR[cur], vb1, sL[1 - cur], -1, NULL);
         pmo_viewbuf_clear(vb2, W, H);
         pmo_propagate(texR, texL, W, H, radius, cw, PMO_COST_CENSUS,
                       16.0f, dmax, theta * 50.0f, 0,
-                      it, sR[cur], smR, vb1, sL[1 - cur], vb2, sR[1 - cur]);
+                      it, sR[cur], smR, vb1, sL[1 - cur], vb2, sR[1 - cur], -1, NULL);
         pmo_normal_depth(sL[1 - cur], W, H, 1.0f / dmax, 1, nd);
         if (pmo_rof(nd, dtL, lambda, 40, W, H, smL)) return -1;
         pmo_normal_depth(sR[1 - cur], W, H, 1.0f / dmax, 0, nd);
@@ -1641,11 +1665,11 @@ int pmo_run_depthmap(const float* left, const float* right, int W, int H,
         pmo_viewbuf_clear(vb1, W, H);
         pmo_propagate(texL, texR, W, H, radius, cw, PMO_COST_CENSUS,
                       16.0f, dmax, theta * 10.0f, 1,
-                      it, sL[cur], smL, vb2, sR[cur], vb1, sL[1 - cur]);
+                      it, sL[cur], smL, vb2, sR[cur], vb1, sL[1 - cur], -1, NULL);
         pmo_viewbuf_clear(vb2, W, H);
         pmo_propagate(texR, texL, W, H, radius, cw, PMO_COST_CENSUS,
                       16.0f, dmax, theta * 10.0f, 0,
-                      it, sR[cur], smR, vb1, sL[1 - cur], vb2, sR[1 - cur]);
+                      it, sR[cur], smR, vb1, sL[1 - cur], vb2, sR[1 - cur], -1, NULL);
         pmo_fill_state(sL[1 - cur], W, H, 1.0f / dmax, 1, nd);
         if (pmo_huber(nd, dtL, lambda, 1e-4f, 40, W, H, smL)) return -1;
         pmo_fill_state(sR[1 - cur], W, H, 1.0f / dmax, 0, nd);

@@ -106,12 +106,17 @@ void pmo_init_prior(const float* tex1, const float* tex2, int W, int H,
 void pmo_viewbuf_clear(uint64_t* vbuf, int W, int H);
 /* pmhstereo_propagate_view, Jacobi schedule (pmhstereo.cl:880-1071).
  * src_other: the other view's state field the keys in viewin refer to. */
+/* color >= 0: one pass of the red-black schedule (not in the reference):
+ * only the pixels with (x + y) & 1 == color; pass 1 reads its colour-0
+ * random neighbours from `fresh` (pass 0's output), the rest from `old`.
+ * color = -1: every pixel (fresh unused). */
 void pmo_propagate(const float* tex1, const float* tex2, int W, int H,
                    int radius, float color_weight, int cost_kind,
                    float cost_weight, float dmax, float theta, int lr,
                    int iter, const float* old, const float* smooth,
                    const uint64_t* viewin, const float* src_other,
-                   uint64_t* viewout, float* out);
+                   uint64_t* viewout, float* out, int color,
+                   const float* fresh);
 /* the view scatter alone (inserts into vbuf, which is not cleared) */
 void pmo_scatter(const float* states, int W, int H, int lr, uint64_t* vbuf);
 /* pmhstereo_consistency (pmhstereo.cl:1095-1129) */
@@ -136,11 +141,13 @@ void pmo_disparity_to_normal(const float* disp, int W, int H,
  * stateL/stateR receive the final plane fields, disp the left disparity,
  * cost_trace (may be NULL) the per-iteration per-pixel cost of both views:
  * layout [iter][view][H][W]. */
+#define PMO_SCHEDULE_JACOBI 0     /* the reference's sweep */
+#define PMO_SCHEDULE_RED_BLACK 1  /* two checkerboard passes per propagate */
 int pmo_run(const float* left, const float* right, int W, int H,
             int channels, int stride, int radius, float dmax, int iters,
-            int cost_kind, float cost_weight, float maxdispdiff,
-            float maxdeg, float* stateL, float* stateR, float* disp,
-            float* cost_trace);
+            int cost_kind, float cost_weight, int schedule,
+            float maxdispdiff, float maxdeg, float* stateL, float* stateR,
+            float* disp, float* cost_trace);
 
 /* ---- PM-Huber smoothing coupling (theta > 0) ---------------------------
  * PDDiffTensorColor / PDDiffTensorGray (cvt/cl/kernel/PDDiffTensor.cl:

@@ -60,6 +60,7 @@ def test_params_default_are_reference_constants(native):
     assert p.color_weight == 12.0        # CENSUS_COLOR_WEIGHT
     assert p.cost_kind == native.PM_COST_CENSUS     # patch_eval_census
     assert p.cost_weight == 16.0         # patch_eval_color_grad_weighted
+    assert p.schedule == native.PM_SCHEDULE_JACOBI  # the reference's sweep
     assert p.smoothing == native.PM_SMOOTH_NONE     # plain PatchMatch
     assert p.theta_min == np.float32(1e-4)          # PMHuberStereo.cpp:92
     assert p.schedule_iterations == 0
@@ -72,6 +73,7 @@ def test_params_default_are_reference_constants(native):
     ("color_weight", -1.0), ("theta_min", -0.5), ("theta_min", float("nan")),
     ("smoothing", 3), ("schedule_iterations", -1), ("device", -1),
     ("cost_kind", 2), ("cost_kind", -1), ("cost_weight", -1.0),
+    ("schedule", 2), ("schedule", -1),
     ("cost_weight", float("inf"))])
 def test_create_rejects_bad_parameters(native, field, value):
     L = native.lib()

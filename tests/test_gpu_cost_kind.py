@@ -153,3 +153,35 @@ def test_full_size_eight_iterations_census(oracle, synth):
 
 def test_full_size_eight_iterations_color_grad(oracle, synth):
     full_size_eight_iterations(oracle, synth, CG, KW)
+
+
+@pytest.mark.parametrize("cost_kind,kw", [(cvt_amd.PM_COST_CENSUS, 16.0),
+                                          (CG, KW)])
+@pytest.mark.parametrize("W,H,C,r", [(157, 83, 4, 4), (300, 61, 1, 5)])
+def test_red_black_schedule_bitexact(oracle, synth, cost_kind, kw, W, H, C,
+                                     r):
+    """PM_SCHEDULE_RED_BLACK (not in the reference) against the oracle's
+    red-black run: per-iteration costs, the final fields and the
+    LR-checked disparity, bit for bit.  The widths leave a partial block of
+    128 columns (a pass's block covers every other column of 128)."""
+    L, R, _ = synth.make_pair(W, H, cells=6, dmax=24.0, channels=C, seed=W)
+    iters = 3
+    ref = oracle.run(L, R, r, 24.0, iters, maxdisp=1.0, maxdeg=180.0,
+                     trace=True, cost_kind=cost_kind, cost_weight=kw,
+                     schedule=1)
+    with cvt_amd.Context(radius=r, disparity_max=24.0, cost_kind=cost_kind,
+                         cost_weight=kw,
+                         schedule=cvt_amd.PM_SCHEDULE_RED_BLACK) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        for it in range(iters):
+            ctx.iterate(1)
+            for v in (0, 1):
+                got = ctx.planes(v)[..., 3]
+                want = ref["cost_trace"][it, v]
+                assert same(got, want), (it, v, first_diff(got, want))
+        for view, key in ((0, "stateL"), (1, "stateR")):
+            got = ctx.planes(view)
+            assert same(got, ref[key]), (view, first_diff(got, ref[key]))
+        d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 180.0)
+    assert same(d, ref["disp"]), first_diff(d, ref["disp"])

@@ -95,13 +95,19 @@ def view_sources(O, src, lr_src, y, x_target):
 
 
 def propagate_py(O, t1, t2, R, dmax, lr, it, old, src_other,
-                 draw_start=prop_draw_start, cost=None):
+                 draw_start=prop_draw_start, cost=None, color=-1, fresh=None,
+                 out=None):
+    """color 0 / 1: one pass of the red-black schedule, the pixels with
+    (x + y) & 1 == color only; pass 1 takes its colour-0 random neighbours
+    from `fresh` (pass 0's output)."""
     cost = cost or {}
     H, W = old.shape[:2]
-    out = np.zeros_like(old)
+    out = np.zeros_like(old) if out is None else out
     theta = F(0.0)
     for y in range(H):
         for x in range(W):
+            if color >= 0 and (x + y) % 2 != color:
+                continue
             p = y * W + x
             u = O.draws(draw_start(p, it), PROP_DRAWS)
             self_ = old[y, x].astype(F).copy()
@@ -133,7 +139,8 @@ def propagate_py(O, t1, t2, R, dmax, lr, it, old, src_other,
                          + F(0.5))
                 nx = min(max(x + ox, 0), W - 1)
                 ny = min(max(y + oy, 0), H - 1)
-                self_ = attempt(old[ny, nx])
+                src = fresh if (color == 1 and (nx + ny) % 2 == 0) else old
+                self_ = attempt(src[ny, nx])
             # random plane
             self_ = attempt(O.state_init(u[8:11], x, y, lr, dmax))
             # other view: viewprop of the source plane
@@ -180,6 +187,62 @@ def test_propagate_matches_sequential_statement(oracle, synth, channels,
     nl2, _ = O.propagate(tl, tr, R, dmax, 1, 1, nl, vbr, nr, **cost)
     pl2 = propagate_py(O, tl, tr, R, dmax, 1, 1, nl, nr, cost=cost)
     np.testing.assert_array_equal(bits(nl2), bits(pl2))
+
+
+@pytest.mark.parametrize("kind", sorted(COSTS))
+def test_red_black_passes_match_sequential_statement(oracle, synth, kind):
+    """The red-black schedule (not in the reference): pass 0 over the
+    pixels with x + y even, then pass 1 over the odd ones reading pass 0's
+    new planes at its even random neighbours; both passes insert into one
+    scatter buffer."""
+    O = oracle
+    cost = COSTS[kind]
+    R, dmax = 2, 5.0
+    L, Rt, _ = small_pair(synth, seed=31)
+    tl, tr = O.prepare(L, R), O.prepare(Rt, R)
+    sl = O.init(tl, tr, R, dmax, 1, **cost)
+    sr = O.init(tr, tl, R, dmax, 0, **cost)
+    H, W = sl.shape[:2]
+    out = np.zeros_like(sl)
+    vb = O.viewbuf_empty(W, H)
+    for color in (0, 1):
+        O.propagate(tl, tr, R, dmax, 1, 1, sl, O.scatter(sr, 0), sr,
+                    color=color, fresh=out, out=out, viewout=vb, **cost)
+    want = propagate_py(O, tl, tr, R, dmax, 1, 1, sl, sr, cost=cost, color=0)
+    want = propagate_py(O, tl, tr, R, dmax, 1, 1, sl, sr, cost=cost, color=1,
+                        fresh=want, out=want)
+    np.testing.assert_array_equal(bits(out), bits(want))
+    # the scatter of both passes is the scatter of the whole new field
+    np.testing.assert_array_equal(vb, O.scatter(out, 1))
+    # and the schedule differs from the Jacobi sweep
+    jac, _ = O.propagate(tl, tr, R, dmax, 1, 1, sl, O.scatter(sr, 0), sr,
+                         **cost)
+    assert (bits(jac) != bits(out)).any()
+
+
+def test_red_black_run_invariants(oracle, synth):
+    """theta = 0 with the red-black schedule over whole runs: every
+    iteration's stored costs equal fresh evaluations and never rise, and
+    the run differs from the Jacobi run."""
+    O = oracle
+    L, Rt, _ = synth.make_pair(48, 32, cells=4, dmax=16.0, channels=4,
+                               seed=13)
+    prev = None
+    for iters in (1, 2, 3):
+        r = O.run(L, Rt, 3, 16.0, iters, trace=True, schedule=1)
+        tl, tr = O.prepare(L, 3), O.prepare(Rt, 3)
+        np.testing.assert_array_equal(
+            bits(O.cost_field(tl, tr, 3, r["stateL"])),
+            bits(r["stateL"][..., 3]))
+        if prev is not None:
+            np.testing.assert_array_equal(bits(r["cost_trace"][:-1]),
+                                          bits(prev["cost_trace"]))
+        tr_ = r["cost_trace"]
+        if iters > 1:
+            assert (tr_[1:] <= tr_[:-1]).all()
+        prev = r
+    jac = O.run(L, Rt, 3, 16.0, 3)
+    assert (bits(jac["stateL"]) != bits(prev["stateL"])).any()
 
 
 def test_statement_detects_a_wrong_draw_offset(oracle, synth):
