@@ -36,8 +36,21 @@ class PmParams(ctypes.Structure):
                 ("stream", ctypes.c_void_p)]
 
 
-# name -> (restype, argtypes); every symbol include/pm_stereo.h and
-# include/pm_io.h declare
+class PmBatchPair(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("channels", ctypes.c_int), ("stride", ctypes.c_int),
+                ("disparity_max", ctypes.c_float),
+                ("d_left", ctypes.c_void_p), ("d_right", ctypes.c_void_p),
+                ("d_gt", ctypes.c_void_p), ("gt_stride", ctypes.c_int)]
+
+
+class PmBatchTuning(ctypes.Structure):
+    _fields_ = [("tile_rows", ctypes.c_int), ("workgroups", ctypes.c_int),
+                ("ms", ctypes.c_float)]
+
+
+# name -> (restype, argtypes); every symbol the headers under include/
+# declare
 SIGNATURES = {
     "pm_params_default": (None, [ctypes.POINTER(PmParams)]),
     "pm_create": (_i, [ctypes.POINTER(PmParams), ctypes.POINTER(_p)]),
@@ -78,6 +91,22 @@ SIGNATURES = {
     "pm_net_train_step": (_i, [_p, _p, _p, _i, _i, _i, _f, _f, _f,
                                ctypes.POINTER(_f)]),
     "pm_setup_proposal": (_i, [_p, _p]),
+    # include/pm_batch.h
+    "pm_batch_create": (_i, [ctypes.POINTER(PmParams), _p, _i,
+                             ctypes.POINTER(_p)]),
+    "pm_batch_destroy": (None, [_p]),
+    "pm_batch_last_error": (ctypes.c_char_p, [_p]),
+    "pm_batch_size": (_i, [_p]),
+    "pm_batch_set_pair": (_i, [_p, _i, _p]),
+    "pm_batch_run": (_i, [_p, _i, _i, _f, _f, _f, _f]),
+    "pm_batch_get_disparity": (_i, [_p, _i, _p]),
+    "pm_batch_get_planes": (_i, [_p, _i, _i, _p]),
+    "pm_batch_get_error_counts": (_i, [_p, _p]),
+    "pm_batch_disparity_ptr": (_i, [_p, _i, ctypes.POINTER(_p)]),
+    "pm_batch_autotune": (_i, [_p, _i, _p, _i, ctypes.POINTER(_i)]),
+    "pm_batch_get_tuning": (_i, [_p, _p]),
+    "pm_batch_set_tuning": (_i, [_p, _p]),
+    "pm_batch_sync": (_i, [_p]),
     # include/pm_io.h
     "pm_io_last_error": (ctypes.c_char_p, []),
     "pm_pfm_info": (_i, [ctypes.c_char_p, _p, _p, _p]),

@@ -179,6 +179,73 @@ hipError_t launch_eval_error(hipStream_t st, const float* disp, int sd,
                              float abs_thres, float rel_thres,
                              unsigned long long* counts);
 
+// ---------------------------------------------------------------------
+// Batches of stereo pairs of uneven sizes (pm_batch.cpp).  Every pair is
+// laid out as a single-pair context lays out its views; a launch covers
+// all pairs, reading each pair's geometry and buffers from a descriptor
+// in device memory.
+// ---------------------------------------------------------------------
+struct BatchPair {
+  Geometry g;               // full image: by0 = 0, nby = 0, color = -1
+  ViewBufs v[2];
+  float4* state[2][2];      // [view][ping-pong]
+  unsigned long long* vbuf[2];
+  float4* checked;          // LR-checked left field
+  float* disp;              // disparity out (W x H)
+  const float* img[2];      // input images, `stride` floats per row
+  const float* gt;          // ground truth for the error counts (may be null)
+  unsigned long long* counts;  // 3 x u64: error, valid, all
+  int channels, stride, gt_stride;
+  float dmax;
+  int nbx;                  // tile blocks per row (64 columns each)
+  long long item0;          // first tile block of the pair in the batch
+};
+
+// tile blocks of 64 x rows pixels of a W x H image
+__host__ __device__ inline long long batch_blocks(int W, int H, int rows) {
+  return (long long)((W + 63) / 64) * ((H + rows - 1) / rows);
+}
+
+// Launch shape of the batch's tile kernels: tile rows (2, 4 or 8) and the
+// work-group count of the persistent grid (each work-group walks tile
+// blocks of its XCD's eighth of the batch; 0 = one work-group per block).
+struct BatchShape {
+  int rows = 4;
+  int workgroups = 0;
+};
+
+constexpr int kBatchRowChoices[3] = {2, 4, 8};
+
+// per-pixel kernels over all pairs: grid z = pair (and view); maxW / maxH
+// bound the pairs' sizes
+hipError_t launch_prepare_batch(hipStream_t st, const BatchPair* pairs,
+                                int npairs, int maxW, int maxH, int R,
+                                float cw);
+hipError_t launch_viewbuf_clear_batch(hipStream_t st, const BatchPair* pairs,
+                                      int npairs, int maxW, int maxH,
+                                      int view);
+hipError_t launch_init_batch(hipStream_t st, const BatchPair* pairs,
+                             int npairs, long long nitems, int view,
+                             int cost_kind, size_t lds, BatchShape shape);
+// one view's propagate of iteration `iter` over all pairs, reading
+// state[view][cur] and writing state[view][1 - cur]
+hipError_t launch_propagate_batch(hipStream_t st, const BatchPair* pairs,
+                                  int npairs, long long nitems, int view,
+                                  int cur, int iter, int cost_kind,
+                                  size_t lds, BatchShape shape);
+// pp: 0 none, 1 LR check, 2 LR check + fill (pm_postproc)
+hipError_t launch_postprocess_batch(hipStream_t st, const BatchPair* pairs,
+                                    int npairs, int maxW, int maxH, int cur,
+                                    int pp, float maxdispdiff, float maxdeg);
+hipError_t launch_eval_error_batch(hipStream_t st, const BatchPair* pairs,
+                                   int npairs, int maxW, int maxH,
+                                   float abs_thres, float rel_thres);
+// LDS bytes of a tile work-group of `rows` rows at patch radius R
+size_t batch_lds_bytes(int R, int cost_kind, int rows);
+// propagate work-groups of that shape one CU holds at once
+hipError_t batch_resident_per_cu(int cost_kind, int rows, size_t lds,
+                                 int* out);
+
 hipError_t launch_draws(hipStream_t st, uint64_t k, int n, float* out);
 
 hipError_t launch_math(hipStream_t st, const float* in, int n, float* out);

@@ -138,25 +138,25 @@ struct Lane {
   const float4* __restrict__ tex2;  // other view's colour record (0, 0)
 };
 
-// Block prologue shared by the cost-evaluating kernels: stages the own-view
-// tile (r, g, b, gradient sign bits), and for kGrad the tile of its
-// gradients after it, and returns the lane context.  For kRB (a red-black
-// pass, g.color = 0 or 1) the block covers 2 x 64 columns and lane i takes
+// Block prologue shared by the cost-evaluating kernels, for the block
+// (bx, by) of tile blocks of 64 x kRows pixels: stages the own-view tile
+// (r, g, b, gradient sign bits), and for kGrad the tile of its gradients
+// after it, and returns the lane context.  For kRB (a red-black pass,
+// g.color = 0 or 1) the block covers 2 x 64 columns and lane i takes
 // column 2 i + ((y + color) & 1) of its row, the pixel of the pass's
-// colour.  Must be reached by every thread of the block.
-template <bool kGrad, bool kRB = false>
-__device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
-                            const ViewBufs& other, float4* smem) {
-  int bx, by;
-  xcd_block(&bx, &by);
-  by += g.by0;
+// colour.  Must be reached by every thread of the block; the block is
+// 64 x kRows threads.
+template <bool kGrad, bool kRB, int kRows>
+__device__ Lane block_setup_at(const Geometry& g, const ViewBufs& self,
+                               const ViewBufs& other, float4* smem, int bx,
+                               int by) {
   const int R = g.R, W = g.W, H = g.H;
   const int cols = kRB ? 2 * kTX : kTX;
-  const int TW = cols + 2 * R, TH = kTY + 2 * R;
-  const int x0 = bx * cols, y0 = by * kTY;
+  const int TW = cols + 2 * R, TH = kRows + 2 * R;
+  const int x0 = bx * cols, y0 = by * kRows;
   const int tid = threadIdx.y * kTX + threadIdx.x;
   const size_t gplane = (size_t)g.TS * H;
-  for (int i = tid; i < TW * TH; i += kTX * kTY) {
+  for (int i = tid; i < TW * TH; i += kTX * kRows) {
     const int gx = x0 - R + i % TW, gy = y0 - R + i / TW;
     float4 v = mk4(0.0f, 0.0f, 0.0f, 0.0f), gv = v;
     if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
@@ -198,6 +198,18 @@ __device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
                ? self.tex[(size_t)L.y * g.TS + kTexPadL + L.x].w
                : 1.0f;
   return L;
+}
+
+// The same for the single-image launches: 64 x kTY blocks, the block from
+// the hardware work-group id in XCD order, offset by the launch's first
+// block row.
+template <bool kGrad, bool kRB = false>
+__device__ Lane block_setup(const Geometry& g, const ViewBufs& self,
+                            const ViewBufs& other, float4* smem) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  return block_setup_at<kGrad, kRB, kTY>(g, self, other, smem, bx,
+                                         by + g.by0);
 }
 
 // Linear sample (CLK_FILTER_LINEAR, CLK_ADDRESS_CLAMP_TO_EDGE,
@@ -522,13 +534,12 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
 // ---------------------------------------------------------------------
 constexpr int kBX = 64, kBY = 4;  // simple per-pixel kernels
 
-__global__ void __launch_bounds__(256)
-k_prepare(const float* __restrict__ img, int W, int H, int C, int stride,
-          int R, float cw, float4* __restrict__ tex,
-          uint8_t* __restrict__ sgn) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= W || y >= H) return;
+__device__ __forceinline__ void prepare_pixel(const float* __restrict__ img,
+                                              int W, int H, int C, int stride,
+                                              int R, float cw,
+                                              float4* __restrict__ tex,
+                                              uint8_t* __restrict__ sgn,
+                                              int x, int y) {
   const size_t p = (size_t)y * W + x;
   const float4 c = fetch_rgba(img, C, stride, x, y);
   float wsum = 0.0f;
@@ -567,13 +578,25 @@ k_prepare(const float* __restrict__ img, int W, int H, int C, int stride,
   sgn[p] = (uint8_t)signmask4(grad);
 }
 
-// pmhstereo_init (pmhstereo.cl:755-773): RNG_init(p, 3), three draws
-template <int kKind>
 __global__ void __launch_bounds__(256)
-k_init(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
-       float4* __restrict__ states) {
-  extern __shared__ float4 smem[];
-  const Lane L = block_setup<kKind == kCostColorGrad>(g, self, other, smem);
+k_prepare(const float* __restrict__ img, int W, int H, int C, int stride,
+          int R, float cw, float4* __restrict__ tex,
+          uint8_t* __restrict__ sgn) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  prepare_pixel(img, W, H, C, stride, R, cw, tex, sgn, x, y);
+}
+
+// pmhstereo_init (pmhstereo.cl:755-773): RNG_init(p, 3), three draws; one
+// tile block of 64 x kRows pixels, reached by every thread of the block
+template <int kKind, int kRows>
+__device__ __forceinline__ void init_block(const Geometry& g, ViewBufs self,
+                                           ViewBufs other, float dmax, int lr,
+                                           float4* __restrict__ states,
+                                           float4* smem, int bx, int by) {
+  const Lane L = block_setup_at<kKind == kCostColorGrad, false, kRows>(
+      g, self, other, smem, bx, by);
   float4 s = mk4(0.0f, 0.0f, 0.0f, 0.0f);
   if (L.valid) {
     Mwc rng = mwc_jump((uint64_t)L.p * 3u);
@@ -585,6 +608,17 @@ k_init(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
     s.w = w;
     states[L.p] = s;
   }
+}
+
+template <int kKind>
+__global__ void __launch_bounds__(256)
+k_init(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
+       float4* __restrict__ states) {
+  extern __shared__ float4 smem[];
+  int bx, by;
+  xcd_block(&bx, &by);
+  init_block<kKind, kTY>(g, self, other, dmax, lr, states, smem, bx,
+                         by + g.by0);
 }
 
 // pmhstereo_disparity_to_normal (pmhstereo.cl:1604-1637)
@@ -751,20 +785,19 @@ __device__ __forceinline__ void scatter(float4 self, int x, int y, int W,
   }
 }
 
+// One tile block of pmhstereo_propagate_view: the pixels of the block
+// (bx, by) of 64 x kRows tiles (kRB: of 128 x kRows, one colour).  Must be
+// reached by every thread of the block.
 // kCount: accumulate the cost taps (counters[0]) and KLT taps
-// (counters[1]) this launch evaluates; the results are identical.
-// Built for five wavefronts per SIMD: 96 VGPRs, with a few values spilled
-// to scratch.  Measured on the 720p pair against the unconstrained build
-// (107 VGPRs, four wavefronts): 8 iterations 52.1 -> 51.8 ms; six
-// wavefronts (80 VGPRs, more spills) gain nothing.
+// (counters[1]) this block evaluates; the results are identical.
 // kRB: one pass of the red-black schedule (g.color); pass 1 reads the
 // neighbours of colour 0 from a.fresh, the planes pass 0 just wrote.
-template <int kKind, bool kCount, bool kRB>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
-k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
-  extern __shared__ float4 smem[];
-  const Lane L =
-      block_setup<kKind == kCostColorGrad, kRB>(g, a.self, a.other, smem);
+template <int kKind, bool kCount, bool kRB, int kRows>
+__device__ __forceinline__ void propagate_block(
+    const Geometry& g, const PropagateArgs& a, unsigned long long* counters,
+    float4* smem, int bx, int by) {
+  const Lane L = block_setup_at<kKind == kCostColorGrad, kRB, kRows>(
+      g, a.self, a.other, smem, bx, by);
   const int W = g.W, H = g.H;
   uint32_t ncost = 0;
   uint32_t* nc = kCount ? &ncost : nullptr;
@@ -863,6 +896,21 @@ k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
   }
 }
 
+// The single-image launch: one tile block per work-group.  Built for five
+// wavefronts per SIMD: 96 VGPRs, with a few values spilled to scratch.
+// Measured on the 720p pair against the unconstrained build (107 VGPRs,
+// four wavefronts): 8 iterations 52.1 -> 51.8 ms; six wavefronts (80
+// VGPRs, more spills) gain nothing.
+template <int kKind, bool kCount, bool kRB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
+k_propagate(Geometry g, PropagateArgs a, unsigned long long* counters) {
+  extern __shared__ float4 smem[];
+  int bx, by;
+  xcd_block(&bx, &by);
+  propagate_block<kKind, kCount, kRB, kTY>(g, a, counters, smem, bx,
+                                           by + g.by0);
+}
+
 // view scatter alone, from a stored field: rebuilds the buffer the field's
 // last propagate left behind
 __global__ void __launch_bounds__(256)
@@ -896,14 +944,10 @@ k_klt_field(Geometry g, ViewBufs self, ViewBufs other,
   if (L.valid) out[L.p] = s;
 }
 
-// pmhstereo_consistency (pmhstereo.cl:1095-1129)
-__global__ void __launch_bounds__(256)
-k_consistency(int W, int H, const float4* __restrict__ left,
-              const float4* __restrict__ right, float maxdispdiff,
-              float maxdeg, float4* __restrict__ out) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= W || y >= H) return;
+// pmhstereo_consistency (pmhstereo.cl:1095-1129) at pixel (x, y)
+__device__ __forceinline__ void consistency_pixel(
+    int W, const float4* __restrict__ left, const float4* __restrict__ right,
+    float maxdispdiff, float maxdeg, float4* __restrict__ out, int x, int y) {
   const size_t p = (size_t)y * W + x;
   const float athr = maxdeg / 180.0f;
   const float4 sl = left[p];
@@ -931,13 +975,20 @@ k_consistency(int W, int H, const float4* __restrict__ left,
   out[p] = val;
 }
 
-// pmhstereo_to_disparity (pmhstereo.cl:1232-1257)
 __global__ void __launch_bounds__(256)
-k_to_disparity(int W, int H, const float4* __restrict__ states, float scale,
-               int lr, float* __restrict__ disp) {
+k_consistency(int W, int H, const float4* __restrict__ left,
+              const float4* __restrict__ right, float maxdispdiff,
+              float maxdeg, float4* __restrict__ out) {
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
   if (x >= W || y >= H) return;
+  consistency_pixel(W, left, right, maxdispdiff, maxdeg, out, x, y);
+}
+
+// pmhstereo_to_disparity (pmhstereo.cl:1232-1257) at pixel (x, y)
+__device__ __forceinline__ void to_disparity_pixel(
+    int W, const float4* __restrict__ states, float scale, int lr,
+    float* __restrict__ disp, int x, int y) {
   const size_t p = (size_t)y * W + x;
   const float4 s = states[p];
   const float d = (transform(s, (float)x, (float)y) - (float)x) *
@@ -948,6 +999,15 @@ k_to_disparity(int W, int H, const float4* __restrict__ states, float scale,
   else
     val = fmaxf(d, 0.0f);
   disp[p] = val * scale;
+}
+
+__global__ void __launch_bounds__(256)
+k_to_disparity(int W, int H, const float4* __restrict__ states, float scale,
+               int lr, float* __restrict__ disp) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  to_disparity_pixel(W, states, scale, lr, disp, x, y);
 }
 
 // row scans of pmhstereo_fill_depthmap / pmhstereo_fill_normalmap
@@ -967,13 +1027,10 @@ __device__ __forceinline__ void row_fill_pair(const float4* __restrict__ in,
   *r = b;
 }
 
-// pmhstereo_fill_depthmap (pmhstereo.cl:1187-1230)
-__global__ void __launch_bounds__(256)
-k_fill_depthmap(int W, int H, const float4* __restrict__ states, float scale,
-                float* __restrict__ disp) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= W || y >= H) return;
+// pmhstereo_fill_depthmap (pmhstereo.cl:1187-1230) at pixel (x, y)
+__device__ __forceinline__ void fill_depthmap_pixel(
+    int W, const float4* __restrict__ states, float scale,
+    float* __restrict__ disp, int x, int y) {
   const size_t p = (size_t)y * W + x;
   const float4 s = states[p];
   float val;
@@ -985,6 +1042,15 @@ k_fill_depthmap(int W, int H, const float4* __restrict__ states, float scale,
     val = fabsf(transform(s, (float)x, (float)y) - (float)x);
   }
   disp[p] = val * scale;
+}
+
+__global__ void __launch_bounds__(256)
+k_fill_depthmap(int W, int H, const float4* __restrict__ states, float scale,
+                float* __restrict__ disp) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  fill_depthmap_pixel(W, states, scale, disp, x, y);
 }
 
 // pmhstereo_fill_normalmap (pmhstereo.cl:1259-1305)
@@ -1012,12 +1078,12 @@ k_fill_normalmap(int W, int H, const float4* __restrict__ states,
 // the estimate is > 0, counts[0] += 1 when |gt - est| > abs_thres and
 // |gt - est| / |gt| > rel_thres.  A block sums its pixels' counts, one
 // atomic per counter.
-__global__ void __launch_bounds__(256)
-k_eval_error(const float* __restrict__ disp, int sd, const float* __restrict__ gt,
-             int sg, int W, int H, float abs_thres, float rel_thres,
-             unsigned long long* __restrict__ counts) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
+// Called by every lane of the wavefront (pixels outside the image count
+// nothing).
+__device__ __forceinline__ void eval_error_pixel(
+    const float* __restrict__ disp, int sd, const float* __restrict__ gt,
+    int sg, int W, int H, float abs_thres, float rel_thres,
+    unsigned long long* __restrict__ counts, int x, int y) {
   unsigned e = 0, v = 0, a = 0;
   if (x < W && y < H) {
     const float g = gt[(size_t)y * sg + x];
@@ -1037,6 +1103,15 @@ k_eval_error(const float* __restrict__ disp, int sd, const float* __restrict__ g
     if (bv) atomicAdd(&counts[1], (unsigned long long)__popcll(bv));
     if (ba) atomicAdd(&counts[2], (unsigned long long)__popcll(ba));
   }
+}
+
+__global__ void __launch_bounds__(256)
+k_eval_error(const float* __restrict__ disp, int sd, const float* __restrict__ gt,
+             int sg, int W, int H, float abs_thres, float rel_thres,
+             unsigned long long* __restrict__ counts) {
+  eval_error_pixel(disp, sd, gt, sg, W, H, abs_thres, rel_thres, counts,
+                   blockIdx.x * kBX + threadIdx.x,
+                   blockIdx.y * kBY + threadIdx.y);
 }
 
 __global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
@@ -1075,6 +1150,145 @@ k_math(const float* __restrict__ in, int n, float* __restrict__ out) {
   out[5 * i + 2] = c;
   out[5 * i + 3] = acospi_f(x);
   out[5 * i + 4] = sqrtf(x);
+}
+
+// ---------------------------------------------------------------------
+// batches of pairs of uneven sizes (BatchPair descriptors in device
+// memory).  The tile kernels number the tile blocks of all pairs in one
+// sequence, pair after pair, row-major within a pair; the per-pixel
+// kernels take the pair (and view) from the grid's z index.
+// ---------------------------------------------------------------------
+
+// Persistent walk over the tile blocks: the sequence is cut into kXcds
+// contiguous eighths, and the work-groups of XCD k (hardware ids congruent
+// to k, dispatched round-robin) take the k-th eighth with a stride of
+// their number, so each XCD's L2 serves a contiguous run of rows.  Each
+// work-group calls body(pair, bx, by) for its blocks in increasing order;
+// every block of [0, nitems) is visited exactly once for any grid of at
+// least kXcds work-groups.  The barrier before each block keeps the
+// previous block's LDS tile intact until every wavefront is done with it.
+template <int kRows, typename Body>
+__device__ __forceinline__ void batch_walk(const BatchPair* __restrict__ pairs,
+                                           int npairs, long long nitems,
+                                           Body body) {
+  const int G = (int)gridDim.x, w = (int)blockIdx.x;
+  const int xcd = w % kXcds, j = w / kXcds;
+  const int g8 = (G - xcd + kXcds - 1) / kXcds;
+  const long long i0 = nitems * xcd / kXcds, i1 = nitems * (xcd + 1) / kXcds;
+  int pi = 0;
+  for (long long it = i0 + j; it < i1; it += g8) {
+    __syncthreads();
+    while (pi + 1 < npairs && pairs[pi + 1].item0 <= it) ++pi;
+    const BatchPair& P = pairs[pi];
+    const long long li = it - P.item0;
+    body(P, (int)(li % P.nbx), (int)(li / P.nbx));
+  }
+}
+
+template <int kKind, int kRows>
+__global__ void __launch_bounds__(64 * kRows)
+k_init_batch(const BatchPair* __restrict__ pairs, int npairs,
+             long long nitems, int view) {
+  extern __shared__ float4 smem[];
+  batch_walk<kRows>(pairs, npairs, nitems,
+                    [&](const BatchPair& P, int bx, int by) {
+    init_block<kKind, kRows>(P.g, P.v[view], P.v[1 - view], P.dmax,
+                             view == 0 ? 1 : 0, P.state[view][0], smem, bx,
+                             by);
+  });
+}
+
+// one view's propagate over all pairs: the single-image launch's
+// arguments (one_iteration in pm_capi.cpp) from each pair's descriptor
+template <int kKind, int kRows>
+__global__ void __launch_bounds__(64 * kRows)
+__attribute__((amdgpu_waves_per_eu(5, 5)))
+k_propagate_batch(const BatchPair* __restrict__ pairs, int npairs,
+                  long long nitems, int view, int cur, int iter) {
+  extern __shared__ float4 smem[];
+  batch_walk<kRows>(pairs, npairs, nitems,
+                    [&](const BatchPair& P, int bx, int by) {
+    PropagateArgs a;
+    a.self = P.v[view];
+    a.other = P.v[1 - view];
+    a.old = P.state[view][cur];
+    a.out = P.state[view][1 - cur];
+    a.smooth = nullptr;
+    a.viewin = P.vbuf[1 - view];
+    a.viewout = P.vbuf[view];
+    // the left view reads the right view's field of the previous
+    // iteration, the right view the left view's new one
+    a.src_other = view == 0 ? P.state[1][cur] : P.state[0][1 - cur];
+    a.dmax = P.dmax;
+    a.theta = 0.0f;
+    a.lr = view == 0 ? 1 : 0;
+    a.iter = iter;
+    propagate_block<kKind, false, false, kRows>(P.g, a, nullptr, smem, bx,
+                                                by);
+  });
+}
+
+__global__ void __launch_bounds__(256)
+k_prepare_batch(const BatchPair* __restrict__ pairs, int R, float cw) {
+  const BatchPair& P = pairs[blockIdx.z >> 1];
+  const int view = blockIdx.z & 1;
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= P.g.W || y >= P.g.H) return;
+  prepare_pixel(P.img[view], P.g.W, P.g.H, P.channels, P.stride, R, cw,
+                P.v[view].tex, P.v[view].sgn, x, y);
+}
+
+__global__ void __launch_bounds__(256)
+k_viewbuf_clear_batch(const BatchPair* __restrict__ pairs, int view) {
+  const BatchPair& P = pairs[blockIdx.z];
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= P.g.W || y >= P.g.H) return;
+  P.vbuf[view][(size_t)y * P.g.W + x] = ~0ull;
+}
+
+__global__ void __launch_bounds__(256)
+k_consistency_batch(const BatchPair* __restrict__ pairs, int cur,
+                    float maxdispdiff, float maxdeg) {
+  const BatchPair& P = pairs[blockIdx.z];
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= P.g.W || y >= P.g.H) return;
+  consistency_pixel(P.g.W, P.state[0][cur], P.state[1][cur], maxdispdiff,
+                    maxdeg, P.checked, x, y);
+}
+
+// pp 0: the left field's disparity; 1: the LR-checked field's; 2: the
+// LR-checked field's, holes filled
+__global__ void __launch_bounds__(256)
+k_disparity_batch(const BatchPair* __restrict__ pairs, int cur, int pp) {
+  const BatchPair& P = pairs[blockIdx.z];
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= P.g.W || y >= P.g.H) return;
+  if (pp == 2)
+    fill_depthmap_pixel(P.g.W, P.checked, 1.0f, P.disp, x, y);
+  else
+    to_disparity_pixel(P.g.W, pp == 1 ? P.checked : P.state[0][cur], 1.0f,
+                       1, P.disp, x, y);
+}
+
+__global__ void __launch_bounds__(256)
+k_zero_counts_batch(const BatchPair* __restrict__ pairs, int npairs) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < 3 * npairs && pairs[i / 3].counts) pairs[i / 3].counts[i % 3] = 0ull;
+}
+
+__global__ void __launch_bounds__(256)
+k_eval_error_batch(const BatchPair* __restrict__ pairs, float abs_thres,
+                   float rel_thres) {
+  const BatchPair& P = pairs[blockIdx.z];
+  if (!P.gt || !P.counts) return;   // uniform over the work-group
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  eval_error_pixel(P.disp, P.g.W, P.gt, P.gt_stride, P.g.W, P.g.H, abs_thres,
+                   rel_thres, P.counts, x, y);
 }
 
 // ---------------------------------------------------------------------
@@ -1276,6 +1490,180 @@ hipError_t launch_eval_error(hipStream_t st, const float* disp, int sd,
   PM_LAUNCH(k_eval_error, dim3((W + kBX - 1) / kBX, (H + kBY - 1) / kBY),
             kBlock2D, 0, st, disp, sd, gt, sg, W, H, abs_thres, rel_thres,
             counts);
+}
+
+// ---------------------------------------------------------------------
+// batch launchers
+// ---------------------------------------------------------------------
+size_t batch_lds_bytes(int R, int cost_kind, int rows) {
+  const size_t tile = (size_t)(kTX + 2 * R) * (rows + 2 * R) * sizeof(float4);
+  return cost_kind == kCostColorGrad ? 2 * tile : tile;
+}
+
+static dim3 batch_pixel_grid(int maxW, int maxH, int z) {
+  return dim3((unsigned)((maxW + kBX - 1) / kBX),
+              (unsigned)((maxH + kBY - 1) / kBY), (unsigned)z);
+}
+
+// persistent grid: at least one work-group per XCD (batch_walk's
+// requirement), at most one per tile block
+static unsigned batch_tile_grid(long long nitems, BatchShape shape) {
+  long long g = shape.workgroups > 0 ? shape.workgroups : nitems;
+  g = std::min(g, nitems);
+  return (unsigned)std::max<long long>(g, kXcds);
+}
+
+hipError_t launch_prepare_batch(hipStream_t st, const BatchPair* pairs,
+                                int npairs, int maxW, int maxH, int R,
+                                float cw) {
+  if (npairs < 1) return hipSuccess;
+  PM_LAUNCH(k_prepare_batch, batch_pixel_grid(maxW, maxH, 2 * npairs),
+            kBlock2D, 0, st, pairs, R, cw);
+}
+
+hipError_t launch_viewbuf_clear_batch(hipStream_t st, const BatchPair* pairs,
+                                      int npairs, int maxW, int maxH,
+                                      int view) {
+  if (npairs < 1) return hipSuccess;
+  PM_LAUNCH(k_viewbuf_clear_batch, batch_pixel_grid(maxW, maxH, npairs),
+            kBlock2D, 0, st, pairs, view);
+}
+
+template <int kKind, int kRows>
+static hipError_t init_batch_rows(hipStream_t st, const BatchPair* pairs,
+                                  int npairs, long long nitems, int view,
+                                  size_t lds, BatchShape shape) {
+  const hipError_t e = allow_lds(k_init_batch<kKind, kRows>, lds);
+  if (e != hipSuccess) return e;
+  PM_LAUNCH((k_init_batch<kKind, kRows>), dim3(batch_tile_grid(nitems, shape)),
+            dim3(kTX, kRows), lds, st, pairs, npairs, nitems, view);
+}
+
+template <int kKind>
+static hipError_t init_batch_kind(hipStream_t st, const BatchPair* pairs,
+                                  int npairs, long long nitems, int view,
+                                  size_t lds, BatchShape shape) {
+  switch (shape.rows) {
+    case 2:
+      return init_batch_rows<kKind, 2>(st, pairs, npairs, nitems, view, lds,
+                                       shape);
+    case 4:
+      return init_batch_rows<kKind, 4>(st, pairs, npairs, nitems, view, lds,
+                                       shape);
+    case 8:
+      return init_batch_rows<kKind, 8>(st, pairs, npairs, nitems, view, lds,
+                                       shape);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_init_batch(hipStream_t st, const BatchPair* pairs,
+                             int npairs, long long nitems, int view,
+                             int cost_kind, size_t lds, BatchShape shape) {
+  if (nitems < 1) return hipSuccess;
+  if (cost_kind == kCostColorGrad)
+    return init_batch_kind<kCostColorGrad>(st, pairs, npairs, nitems, view,
+                                           lds, shape);
+  return init_batch_kind<kCostCensus>(st, pairs, npairs, nitems, view, lds,
+                                      shape);
+}
+
+template <int kKind, int kRows>
+static hipError_t propagate_batch_rows(hipStream_t st, const BatchPair* pairs,
+                                       int npairs, long long nitems, int view,
+                                       int cur, int iter, size_t lds,
+                                       BatchShape shape) {
+  const hipError_t e = allow_lds(k_propagate_batch<kKind, kRows>, lds);
+  if (e != hipSuccess) return e;
+  PM_LAUNCH((k_propagate_batch<kKind, kRows>),
+            dim3(batch_tile_grid(nitems, shape)), dim3(kTX, kRows), lds, st,
+            pairs, npairs, nitems, view, cur, iter);
+}
+
+template <int kKind>
+static hipError_t propagate_batch_kind(hipStream_t st, const BatchPair* pairs,
+                                       int npairs, long long nitems, int view,
+                                       int cur, int iter, size_t lds,
+                                       BatchShape shape) {
+  switch (shape.rows) {
+    case 2:
+      return propagate_batch_rows<kKind, 2>(st, pairs, npairs, nitems, view,
+                                            cur, iter, lds, shape);
+    case 4:
+      return propagate_batch_rows<kKind, 4>(st, pairs, npairs, nitems, view,
+                                            cur, iter, lds, shape);
+    case 8:
+      return propagate_batch_rows<kKind, 8>(st, pairs, npairs, nitems, view,
+                                            cur, iter, lds, shape);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_propagate_batch(hipStream_t st, const BatchPair* pairs,
+                                  int npairs, long long nitems, int view,
+                                  int cur, int iter, int cost_kind,
+                                  size_t lds, BatchShape shape) {
+  if (nitems < 1) return hipSuccess;
+  if (cost_kind == kCostColorGrad)
+    return propagate_batch_kind<kCostColorGrad>(st, pairs, npairs, nitems,
+                                                view, cur, iter, lds, shape);
+  return propagate_batch_kind<kCostCensus>(st, pairs, npairs, nitems, view,
+                                           cur, iter, lds, shape);
+}
+
+template <int kKind, int kRows>
+static hipError_t resident_rows(size_t lds, int* out) {
+  hipError_t e = allow_lds(k_propagate_batch<kKind, kRows>, lds);
+  if (e != hipSuccess) return e;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      out, k_propagate_batch<kKind, kRows>, kTX * kRows, lds);
+}
+
+hipError_t batch_resident_per_cu(int cost_kind, int rows, size_t lds,
+                                 int* out) {
+  const bool cg = cost_kind == kCostColorGrad;
+  switch (rows) {
+    case 2:
+      return cg ? resident_rows<kCostColorGrad, 2>(lds, out)
+                : resident_rows<kCostCensus, 2>(lds, out);
+    case 4:
+      return cg ? resident_rows<kCostColorGrad, 4>(lds, out)
+                : resident_rows<kCostCensus, 4>(lds, out);
+    case 8:
+      return cg ? resident_rows<kCostColorGrad, 8>(lds, out)
+                : resident_rows<kCostCensus, 8>(lds, out);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_postprocess_batch(hipStream_t st, const BatchPair* pairs,
+                                    int npairs, int maxW, int maxH, int cur,
+                                    int pp, float maxdispdiff, float maxdeg) {
+  if (npairs < 1) return hipSuccess;
+  if (pp < 0 || pp > 2) return hipErrorInvalidValue;
+  const dim3 grid = batch_pixel_grid(maxW, maxH, npairs);
+  if (pp != 0) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_consistency_batch, grid, kBlock2D, 0, st, pairs,
+                       cur, maxdispdiff, maxdeg);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  PM_LAUNCH(k_disparity_batch, grid, kBlock2D, 0, st, pairs, cur, pp);
+}
+
+hipError_t launch_eval_error_batch(hipStream_t st, const BatchPair* pairs,
+                                   int npairs, int maxW, int maxH,
+                                   float abs_thres, float rel_thres) {
+  if (npairs < 1) return hipSuccess;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_zero_counts_batch,
+                     dim3((unsigned)((3 * npairs + 255) / 256)), dim3(256), 0,
+                     st, pairs, npairs);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  PM_LAUNCH(k_eval_error_batch, batch_pixel_grid(maxW, maxH, npairs),
+            kBlock2D, 0, st, pairs, abs_thres, rel_thres);
 }
 
 hipError_t launch_draws(hipStream_t st, uint64_t k, int n, float* out) {

@@ -23,6 +23,28 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                        "weights", "proposal.bin")
 _MAGIC = b"PMNETW1\0"
 
+# the network's convolutions (pm_net_capi.cpp, kC): input channels, output
+# channels and resolution level (2^-level of the padded input)
+CONVS = [(8, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 96, 2),
+         (96, 96, 2), (96, 128, 3), (128, 128, 3), (128, 128, 3),
+         (128, 128, 3), (96 + 128, 96, 2), (96, 96, 2), (64 + 96, 64, 1),
+         (64, 64, 1), (32 + 64, 32, 0), (32, 32, 0), (32, 4, 0)]
+
+
+def padded(H, W):
+    """the network's input size for an H x W pair: multiples of 8"""
+    return (H + 7) // 8 * 8, (W + 7) // 8 * 8
+
+
+def forward_flops(H, W, N=1):
+    """multiply-add work of one forward pass over N padded H x W inputs
+    (2 FLOP per multiply-add, 3 x 3 taps; the head's 4 real outputs)"""
+    Hp, Wp = padded(H, W)
+    total = 0
+    for cin, cout, lev in CONVS:
+        total += 2 * 9 * cin * cout * N * (Hp >> lev) * (Wp >> lev)
+    return total
+
 
 class ProposalNet:
     def __init__(self, device=0, stream=None, seed=0, weights=None):

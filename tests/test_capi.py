@@ -1,6 +1,6 @@
 """The C ABI library without a GPU: it loads, exports every entry point
-include/pm_stereo.h, include/pm_io.h and include/pm_net.h declare, its
-headers compile as C and C++, parameters
+include/pm_stereo.h, include/pm_io.h, include/pm_net.h and
+include/pm_batch.h declare, its headers compile as C and C++, parameters
 are validated before any device is touched, and a host without a HIP
 device gets PM_ERR_HIP (no CPU fallback)."""
 import ctypes
@@ -16,12 +16,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pm_stereo.h")
 IO_HEADER = os.path.join(ROOT, "include", "pm_io.h")
 NET_HEADER = os.path.join(ROOT, "include", "pm_net.h")
+BATCH_HEADER = os.path.join(ROOT, "include", "pm_batch.h")
 CXX_HEADER = os.path.join(ROOT, "include", "cvt_amd", "PMHuberStereo.h")
 
 
 def declared_functions():
     names = set()
-    for h in (HEADER, IO_HEADER, NET_HEADER):
+    for h in (HEADER, IO_HEADER, NET_HEADER, BATCH_HEADER):
         text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
         names |= set(re.findall(r"\b(pm_[a-z_0-9]+)\s*\(", text))
     return names
@@ -124,6 +125,54 @@ def test_no_device_is_an_error_not_a_fallback(native):
         cvt_amd.Context(radius=2, disparity_max=8.0)
 
 
+def _batch_params(native, **kw):
+    p = native.PmParams()
+    native.lib().pm_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _batch_pair(native, **kw):
+    q = native.PmBatchPair(width=32, height=16, channels=4, stride=128,
+                           disparity_max=8.0, d_left=1, d_right=1, d_gt=None,
+                           gt_stride=0)
+    for k, v in kw.items():
+        setattr(q, k, v)
+    return q
+
+
+@pytest.mark.parametrize("params,pair", [
+    ({"radius": 0}, {}), ({"smoothing": 1}, {}), ({"schedule": 1}, {}),
+    ({"cost_kind": 2}, {}), ({}, {"width": 0}), ({}, {"channels": 3}),
+    ({}, {"stride": 100}), ({}, {"disparity_max": 0.0}),
+    ({}, {"disparity_max": float("nan")}), ({}, {"d_left": None}),
+    ({}, {"d_gt": 1, "gt_stride": 8})])
+def test_batch_create_rejects_bad_arguments(native, params, pair):
+    L = native.lib()
+    p = _batch_params(native, **params)
+    q = (native.PmBatchPair * 2)(_batch_pair(native),
+                                 _batch_pair(native, **pair))
+    b = ctypes.c_void_p()
+    rc = L.pm_batch_create(ctypes.byref(p), q, 2, ctypes.byref(b))
+    try:
+        assert rc == native.PM_ERR_ARG
+        assert L.pm_batch_last_error(b)
+    finally:
+        L.pm_batch_destroy(b)
+
+
+def test_batch_null_arguments(native):
+    L = native.lib()
+    assert L.pm_batch_create(None, None, 0, None) == native.PM_ERR_ARG
+    assert L.pm_batch_last_error(None) == b"NULL batch"
+    L.pm_batch_destroy(None)
+    assert L.pm_batch_size(None) == -1
+    assert L.pm_batch_sync(None) == native.PM_ERR_ARG
+    assert L.pm_batch_run(None, 1, 1, 1.0, 5.0, -1.0, 0.0) == \
+        native.PM_ERR_ARG
+
+
 def test_missing_library_raises(native, monkeypatch):
     monkeypatch.setattr(native, "_lib", None)
     monkeypatch.setattr(native, "SO_PATH", native.SO_PATH + ".absent")
@@ -135,6 +184,7 @@ C_PROGRAM = r"""
 #include "pm_stereo.h"
 #include "pm_io.h"
 #include "pm_net.h"
+#include "pm_batch.h"
 #include <stdio.h>
 int main(void) {
     pm_params p;
@@ -151,6 +201,19 @@ int main(void) {
     if (pm_net_create(0, NULL, 1, &net) != PM_ERR_HIP) return 3;
     pm_net_destroy(net);
     if (pm_setup_proposal(NULL, NULL) != PM_ERR_ARG) return 4;
+    {   /* a batch checks its arguments before it looks for a device */
+        pm_batch* b = 0;
+        pm_batch_pair q = {16, 8, 4, 64, 4.0f, (const float*)&p,
+                           (const float*)&p, NULL, 0};
+        pm_params_default(&p);
+        if (pm_batch_create(&p, &q, 0, &b) != PM_ERR_ARG) return 5;
+        pm_batch_destroy(b);
+        b = 0;
+        if (pm_batch_create(&p, &q, 1, &b) != PM_ERR_HIP) return 6;
+        pm_batch_destroy(b);
+        if (pm_batch_run(NULL, 1, 1, 1.0f, 5.0f, -1.0f, 0.0f) != PM_ERR_ARG)
+            return 7;
+    }
     return rc == PM_ERR_ARG ? 0 : 1;
 }
 """
