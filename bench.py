@@ -272,6 +272,280 @@ def cpu_baseline_huber(L, R):
             "ms_per_pair": per_pair * 1e3}
 
 
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA peak
+
+
+def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
+                    steps, warmup):
+    """Learned-init PatchMatch on the 720p pair: the proposal network seeds
+    both views (pm_setup_proposal: the network on the pair and on the
+    mirrored pair, each pixel keeping the proposed plane unless the random
+    init plane costs less), then k refine iterations.  bad-1.0px after
+    k = 0..8 refine iterations; k is the fewest that match the plain
+    8-iteration bad-1.0px (or 8 when none does).  Timed like the plain
+    step: set images from HBM, setup, iterations, LR check, disparity.
+    None when no trained weights file exists."""
+    import numpy as np
+    import torch
+
+    import cvt_amd
+    from cvt_amd.net import WEIGHTS, ProposalNet, forward_flops, padded
+    if not os.path.exists(WEIGHTS):
+        return None
+    dD = torch.empty((H, W), dtype=torch.float32, device=dL.device)
+    net = ProposalNet(stream=stream.cuda_stream, weights=WEIGHTS)
+    ctx = cvt_amd.Context(radius=RADIUS, disparity_max=DMAX,
+                          stream=stream.cuda_stream, cost_kind=cost_kind,
+                          cost_weight=cost_weight)
+    bad = []
+    with torch.cuda.stream(stream):
+        for k in range(ITERS + 1):
+            ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
+            ctx.setup_proposal(net)
+            if k:
+                ctx.iterate(k)
+            ctx.disparity_device(dD.data_ptr(), cvt_amd.DISPARITY_PP_LRCHECK,
+                                 MAXDISP, MAXDEG)
+            torch.cuda.synchronize()
+            bad.append(cvt_amd.disparity.bad_pixel_rate(dD.cpu().numpy(), gt,
+                                                        1.0))
+        matched = matched_iterations(bad, plain_bad)
+        k = matched if matched is not None else ITERS
+
+        def step(ev=None):
+            ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
+            if ev:
+                ev[0].record(stream)
+            ctx.setup_proposal(net)
+            if ev:
+                ev[1].record(stream)
+            ctx.iterate(k)
+            if ev:
+                ev[2].record(stream)
+            ctx.disparity_device(dD.data_ptr(), cvt_amd.DISPARITY_PP_LRCHECK,
+                                 MAXDISP, MAXDEG)
+
+        for _ in range(warmup):
+            step()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
+              for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            step(ev[i])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        disp = dD.cpu().numpy()
+        setup_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+        iter_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+        # the network alone: one forward pass over the padded pair (the
+        # setup runs two, the pair and its mirror), best of five
+        Hp, Wp = padded(H, W)
+        x = torch.rand((1, Hp, Wp, 8), device=dL.device)
+        out = torch.empty((1, Hp, Wp, 4), device=dL.device)
+        fwd = []
+        for _ in range(6):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            net.forward(x.data_ptr(), 1, Hp, Wp, out.data_ptr())
+            e1.record(stream)
+            torch.cuda.synchronize()
+            fwd.append(e0.elapsed_time(e1))
+        fwd_ms = float(np.min(fwd[1:]))
+    ctx.close()
+    net.close()
+    flops = forward_flops(H, W)
+    tflops = flops / (fwd_ms * 1e-3) / 1e12
+    return {
+        "value": steps / (t1 - t0), "ms_per_step": (t1 - t0) / steps * 1e3,
+        "refine_iterations": k, "matched": matched is not None,
+        "bad_1px": cvt_amd.disparity.bad_pixel_rate(disp, gt, 1.0),
+        "bad_1px_by_refine_iterations": bad,
+        "setup_ms": setup_ms, "iterations_ms": iter_ms,
+        "weights": os.path.relpath(WEIGHTS, ROOT),
+        "roofline_net": {"bound": "mfma", "kernel": "k_conv (forward pass)",
+                         "achieved": tflops, "peak": BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s",
+                         "frac": tflops / BF16_PEAK_TFLOPS,
+                         "traffic": None,
+                         "forward_ms": fwd_ms, "flop_per_forward": flops,
+                         "forwards_per_pair": 2}}
+
+
+# the batched configuration: 64 pairs of seeded uneven sizes
+BATCH_PAIRS, BATCH_SEED = 64, 3
+
+
+def batched_section(stream, cost_kind, cost_weight, runs=2):
+    """Batched plain PatchMatch (pm_batch): 64 RGBA pairs from the GPU
+    generator, width uniform in 320-1920, height in 240-1080, 24 cells,
+    disparity range 0.1 x width; 8 iterations, LR check at 1 px and the
+    error counts, all pairs in one graph launch per run, after the
+    autotuner picked the launch shape.  Beside it, the same pairs one after
+    another through one single-pair context; every pair's disparity is
+    compared bit for bit."""
+    import numpy as np
+    import torch
+
+    import cvt_amd
+    from cvt_amd import synth_gpu
+    from cvt_amd.batch import Batch
+    rng = np.random.default_rng(BATCH_SEED)
+    sizes = [(int(rng.integers(320, 1921)), int(rng.integers(240, 1081)))
+             for _ in range(BATCH_PAIRS)]
+    data = []
+    for i, (w, h) in enumerate(sizes):
+        L, R, g = synth_gpu.make_pairs(1, w, h, cells=24, dmax=0.1 * w,
+                                       seed=BATCH_SEED * 1000 + i)
+        data.append((L[0], R[0], g[0]))
+    torch.cuda.synchronize()
+    pairs = [{"width": w, "height": h, "channels": 4, "disparity_max": 0.1 * w,
+              "left": L.data_ptr(), "right": R.data_ptr(),
+              "gt": g.data_ptr()}
+             for (w, h), (L, R, g) in zip(sizes, data)]
+    npix = sum(w * h for w, h in sizes)
+    with Batch(pairs, radius=RADIUS, cost_kind=cost_kind,
+               cost_weight=cost_weight, stream=stream.cuda_stream) as b:
+        table = b.autotune(reps=2)
+        tuning = b.tuning
+        b.run(ITERS, cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP, MAXDEG, 1.0, 0.0)
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(runs):
+            b.run(ITERS, cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP, MAXDEG, 1.0,
+                  0.0)
+        b.sync()
+        t1 = time.perf_counter()
+        counts = b.error_counts().astype(np.float64)
+        bdisp = [b.disparity(i) for i in range(len(sizes))]
+    # the same pairs one after another, each through a single-pair context
+    # of its own (its disparity range is a context parameter), after one
+    # untimed pass that sizes every context's buffers and streams
+    outs = [torch.empty((h, w), device="cuda") for w, h in sizes]
+    ctxs = [cvt_amd.Context(radius=RADIUS, disparity_max=0.1 * w,
+                            stream=stream.cuda_stream, cost_kind=cost_kind,
+                            cost_weight=cost_weight) for w, h in sizes]
+
+    def one(i):
+        (w, h), (L, R, _) = sizes[i], data[i]
+        ctxs[i].set_images_device(L.data_ptr(), R.data_ptr(), w, h, 4)
+        ctxs[i].setup()
+        ctxs[i].iterate(ITERS)
+        ctxs[i].disparity_device(outs[i].data_ptr(),
+                                 cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP,
+                                 MAXDEG)
+
+    with torch.cuda.stream(stream):
+        for i in range(len(sizes)):
+            one(i)
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        for i in range(len(sizes)):
+            one(i)
+        torch.cuda.synchronize()
+        s1 = time.perf_counter()
+    for c in ctxs:
+        c.close()
+    identical = all(np.array_equal(bdisp[i].view(np.uint32),
+                                   outs[i].cpu().numpy().view(np.uint32))
+                    for i in range(len(sizes)))
+    bad = counts[:, 0] / counts[:, 2]
+    return {"workload": "batched plain PatchMatch: 64 RGBA f32 pairs, width "
+                        "320-1920, height 240-1080 (seeded), 24 cells, "
+                        "disparity 0.1 x width, 9x9 patch, 8 iterations, LR "
+                        "check 1 px, one graph launch per batch",
+            "value": BATCH_PAIRS * runs / (t1 - t0), "unit": "pairs/s",
+            "ms_per_batch": (t1 - t0) / runs * 1e3,
+            "mpixel_per_s": npix * runs / (t1 - t0) / 1e6,
+            "sequential_single_pair": {
+                "value": BATCH_PAIRS / (s1 - s0), "unit": "pairs/s",
+                "ms_per_batch": (s1 - s0) * 1e3},
+            "speedup_vs_sequential": (s1 - s0) / ((t1 - t0) / runs),
+            "bit_identical_to_single_pair": identical,
+            "bad_1px_mean": float(np.mean(bad)),
+            "bad_1px_pooled": float(counts[:, 0].sum() / counts[:, 2].sum()),
+            "tuning": tuning, "autotune_candidates": table,
+            "pixels": npix, "seed": BATCH_SEED}
+
+
+def matched_iterations(bad, target):
+    """the fewest refine iterations k >= 1 whose bad-1.0px (bad[k]) is at
+    most the target, or None"""
+    return next((k for k in range(1, len(bad)) if bad[k] <= target), None)
+
+
+# the batched serving mix: 16 pairs of seeded uneven sizes
+SERVE_PAIRS, SERVE_SEED, SERVE_REFINE = 16, 5, 2
+
+
+def serving_section(stream, cost_kind, cost_weight, runs=3):
+    """Batched learned serving mix: 16 RGBA pairs from the GPU generator,
+    width uniform in 320-1920, height in 240-1080, disparity range 0.1 x
+    width (up to 192 px); the proposal network seeds every pair and 2
+    refine iterations follow, all in one graph launch, next to plain
+    8-iteration batched PatchMatch on the same pairs.  None without a
+    trained weights file."""
+    import numpy as np
+    import torch
+
+    import cvt_amd
+    from cvt_amd import synth_gpu
+    from cvt_amd.batch import Batch
+    from cvt_amd.net import WEIGHTS, ProposalNet
+    if not os.path.exists(WEIGHTS):
+        return None
+    rng = np.random.default_rng(SERVE_SEED)
+    sizes = [(int(rng.integers(320, 1921)), int(rng.integers(240, 1081)))
+             for _ in range(SERVE_PAIRS)]
+    data = []
+    for i, (w, h) in enumerate(sizes):
+        L, R, g = synth_gpu.make_pairs(1, w, h, cells=24, dmax=0.1 * w,
+                                       seed=SERVE_SEED * 1000 + i)
+        data.append((L[0], R[0], g[0]))
+    torch.cuda.synchronize()
+    pairs = [{"width": w, "height": h, "channels": 4, "disparity_max": 0.1 * w,
+              "left": L.data_ptr(), "right": R.data_ptr(),
+              "gt": g.data_ptr()}
+             for (w, h), (L, R, g) in zip(sizes, data)]
+    out = {}
+    with ProposalNet(stream=stream.cuda_stream, weights=WEIGHTS) as net, \
+            Batch(pairs, radius=RADIUS, cost_kind=cost_kind,
+                  cost_weight=cost_weight, stream=stream.cuda_stream) as b:
+        b.autotune(reps=2)
+        for name, seeded, iters in (("plain", False, ITERS),
+                                    ("learned", True, SERVE_REFINE)):
+            b.set_proposal(net if seeded else None)
+            b.run(iters, cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP, MAXDEG, 1.0,
+                  0.0)
+            b.sync()
+            t0 = time.perf_counter()
+            for _ in range(runs):
+                b.run(iters, cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP, MAXDEG,
+                      1.0, 0.0)
+            b.sync()
+            t1 = time.perf_counter()
+            c = b.error_counts().astype(np.float64)
+            out[name] = {"value": SERVE_PAIRS * runs / (t1 - t0),
+                         "unit": "pairs/s",
+                         "ms_per_batch": (t1 - t0) / runs * 1e3,
+                         "iterations": iters,
+                         "bad_1px_mean": float(np.mean(c[:, 0] / c[:, 2])),
+                         "bad_1px_pooled": float(c[:, 0].sum() /
+                                                 c[:, 2].sum())}
+        tuning = b.tuning
+    return {"workload": "batched serving mix: 16 RGBA f32 pairs, width "
+                        "320-1920, height 240-1080 (seeded), 24 cells, "
+                        "disparity 0.1 x width (<= 192 px), 9x9 patch, LR "
+                        "check 1 px; learned: proposal network + 2 refine "
+                        "iterations, plain: 8 iterations; one graph launch "
+                        "per batch",
+            "learned": out["learned"], "plain": out["plain"],
+            "speedup_vs_plain": out["learned"]["value"] /
+            out["plain"]["value"],
+            "tuning": tuning, "seed": SERVE_SEED}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,6 +567,11 @@ def main():
     ap.add_argument("--io-steps", type=int, default=2,
                     help="pairs run from and to host memory after the timed "
                          "region (PCIe-inclusive rate)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-init run (the line is then plain "
+                         "PatchMatch)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched 64-pair run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the engine's plain copy kernel over 512 "
                          "MiB (calibration run for the HBM byte counters)")
@@ -467,6 +746,48 @@ def main():
             "avg_launch_ms": rof_ms,
             "steps_per_launch": ROF_STEPS_PER_LAUNCH,
             "iteration_ms": float(np.mean(iter_ms))}
+    if not huber and not args.no_learned:
+        log("bench: learned-init PatchMatch (proposal network + refine)")
+        learned = learned_section(dL, dR, gt, stream, cost_kind, cost_weight,
+                                  bad1, args.steps, args.warmup)
+        if learned is None:
+            result["learned"] = {"note": "no proposal weights file (%s); "
+                                         "the line is plain PatchMatch" %
+                                         os.path.relpath(
+                                             cvt_amd.net.WEIGHTS, ROOT)}
+        else:
+            # the headline: learned-init pairs/s at the plain run's bad-1px
+            plain = {"value": value, "ms_per_step": result["ms_per_step"],
+                     "bad_1px": bad1, "iterations": ITERS}
+            k = learned["refine_iterations"]
+            result["value"] = learned["value"]
+            result["ms_per_step"] = learned["ms_per_step"]
+            result["config"]["variant"] = "learned"
+            result["config"]["refine_iterations"] = k
+            result["config"]["workload"] = (
+                "learned-init PatchMatch: bf16 proposal network (4-level "
+                "32-64-96-128 U-Net on the pair and its mirror) seeding both "
+                "views with per-pixel fallback to random init, then %d "
+                "refine iteration%s; 1280x720 RGBA f32 pair, 24 planar "
+                "cells, disparity 0-128 px, 9x9 patch, %s, LR check 1 px; "
+                "against plain PatchMatch with 8 iterations"
+                % (k, "" if k == 1 else "s",
+                   "gamma-10 truncated colour+gradient cost (weight 25.5)"
+                   if args.cost == "color_grad" else "census cost (weight 12)"))
+            result["quality"] = {"bad_1px": learned["bad_1px"],
+                                 "plain_bad_1px": bad1,
+                                 "matched": learned["matched"]}
+            result["plain"] = plain
+            result["learned"] = learned
+            result["speedup_vs_plain"] = learned["value"] / value
+            result["dtype"] = "f32 (PatchMatch), bf16/f32-acc (network)"
+    if not huber and not args.no_batched:
+        log("bench: batched PatchMatch, 64 pairs of uneven sizes")
+        result["batched"] = batched_section(stream, cost_kind, cost_weight)
+        log("bench: batched serving mix, 16 pairs, learned vs plain")
+        serving = serving_section(stream, cost_kind, cost_weight)
+        if serving is not None:
+            result["serving"] = serving
     if not huber and not args.no_schedules:
         log("bench: bad-1px by iteration, Jacobi and red-black schedules")
         result["schedules"] = schedules_by_iteration(dL, dR, gt, cost_kind,

@@ -106,6 +106,7 @@ SIGNATURES = {
     "pm_batch_autotune": (_i, [_p, _i, _p, _i, ctypes.POINTER(_i)]),
     "pm_batch_get_tuning": (_i, [_p, _p]),
     "pm_batch_set_tuning": (_i, [_p, _p]),
+    "pm_batch_set_proposal": (_i, [_p, _p]),
     "pm_batch_sync": (_i, [_p]),
     # include/pm_io.h
     "pm_io_last_error": (ctypes.c_char_p, []),

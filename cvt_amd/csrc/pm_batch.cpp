@@ -21,6 +21,7 @@
 
 #include "pm_batch.h"
 #include "pm_engine.h"
+#include "pm_net_internal.h"
 
 namespace {
 
@@ -34,8 +35,11 @@ struct RunKey {
   int iterations = -1, pp = -1;
   float maxdispdiff = 0.0f, maxdeg = 0.0f, err_abs = -1.0f, err_rel = 0.0f;
   pm::BatchShape shape;
+  const pm_net* net = nullptr;
+  long net_generation = 0;
   bool operator==(const RunKey& o) const {
-    return iterations == o.iterations && pp == o.pp &&
+    return net == o.net && net_generation == o.net_generation &&
+           iterations == o.iterations && pp == o.pp &&
            maxdispdiff == o.maxdispdiff && maxdeg == o.maxdeg &&
            err_abs == o.err_abs && err_rel == o.err_rel &&
            shape.rows == o.shape.rows &&
@@ -62,6 +66,8 @@ struct pm_batch {
   bool ran = false;
   int cur = 0;                       // plane field of the last run's result
   bool counted = false;              // the last run counted errors
+
+  pm_net* net = nullptr;              // proposal seeding (may be null)
 
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -164,15 +170,43 @@ size_t tile_lds(const pm_batch* b, int rows) {
   return pm::batch_lds_bytes(b->prm.radius, b->prm.cost_kind, rows);
 }
 
-// texels and random init of both views, scatter slots of the right view
-// cleared (pm_setup)
+// the proposal-seeded init of every pair (pm_setup_proposal): the
+// network's pass over pair i and its mirror, then the init of both views,
+// in stream order (the next pass overwrites the network's output)
+int enqueue_proposal_init(pm_batch* b, hipStream_t st) {
+  for (int i = 0; i < b->n; ++i) {
+    const pm::BatchPair& P = b->host[i];
+    const float* prop[2] = {nullptr, nullptr};
+    int Wp = 0, os = 0;
+    if (pm::net::proposal(b->net, st, P.img[0], P.img[1], P.g.W, P.g.H,
+                          P.channels, P.stride, prop, &Wp, &os))
+      return fail(b, PM_ERR_HIP, "proposal network, pair %d: %s", i,
+                  pm::net::error_of(b->net));
+    for (int view = 0; view < 2; ++view)
+      PB_HIP(b, pm::launch_init_proposal(st, P.g, P.v[view], P.v[1 - view],
+                                         P.dmax, view == 0 ? 1 : 0,
+                                         prop[view], os, Wp,
+                                         PM_NET_DISP_SCALE,
+                                         P.state[view][0]));
+  }
+  return PM_OK;
+}
+
+// texels and init of both views (random, or seeded by the proposal
+// network), scatter slots of the right view cleared (pm_setup,
+// pm_setup_proposal)
 int enqueue_setup(pm_batch* b, hipStream_t st) {
   const size_t lds = tile_lds(b, b->shape.rows);
   PB_HIP(b, pm::launch_prepare_batch(st, b->dev, b->n, b->maxW, b->maxH,
                                      b->prm.radius, b->prm.color_weight));
-  for (int view = 0; view < 2; ++view)
-    PB_HIP(b, pm::launch_init_batch(st, b->dev, b->n, b->nitems, view,
-                                    b->prm.cost_kind, lds, b->shape));
+  if (b->net) {
+    int rc;
+    if ((rc = enqueue_proposal_init(b, st))) return rc;
+  } else {
+    for (int view = 0; view < 2; ++view)
+      PB_HIP(b, pm::launch_init_batch(st, b->dev, b->n, b->nitems, view,
+                                      b->prm.cost_kind, lds, b->shape));
+  }
   PB_HIP(b, pm::launch_viewbuf_clear_batch(st, b->dev, b->n, b->maxW,
                                            b->maxH, 1));
   return PM_OK;
@@ -246,6 +280,25 @@ int check_index(pm_batch* b, int i) {
 
 int require_run(pm_batch* b) {
   if (!b->ran) return fail(b, PM_ERR_STATE, "no pm_batch_run yet");
+  return PM_OK;
+}
+
+// the net's buffers sized for the largest pair (padded to multiples of 8)
+int reserve_net(pm_batch* b, pm_net* net) {
+  int hp = 8, wp = 8;
+  long long best = 0;
+  for (const pm::BatchPair& P : b->host) {
+    const int h = (P.g.H + 7) / 8 * 8, w = (P.g.W + 7) / 8 * 8;
+    if ((long long)h * w > best) {
+      best = (long long)h * w;
+      hp = h;
+      wp = w;
+    }
+  }
+  if (pm::net::reserve(net, hp, wp))
+    return fail(b, PM_ERR_HIP, "proposal network: %s",
+                pm::net::error_of(net));
+  PB_HIP(b, hipSetDevice(b->prm.device));
   return PM_OK;
 }
 
@@ -417,6 +470,13 @@ int pm_batch_run(pm_batch* b, int iterations, int pp, float maxdispdiff,
   k.err_rel = err_abs >= 0.0f ? err_rel : 0.0f;
   k.shape = b->shape;
   int rc;
+  if (b->net) {
+    // the net may have been used on larger inputs since: size it again,
+    // and record again when that moved its buffers
+    if ((rc = reserve_net(b, b->net))) return rc;
+    k.net = b->net;
+    k.net_generation = pm::net::generation(b->net);
+  }
   if (!b->exec || !(b->key == k))
     if ((rc = record(b, k))) return rc;
   PB_HIP(b, hipGraphLaunch(b->exec, b->stream));
@@ -582,6 +642,24 @@ int pm_batch_autotune(pm_batch* b, int reps, pm_batch_tuning* table,
   if (count) *count = (int)cand.size();
   if ((rc = pm_batch_set_tuning(b, &cand[best]))) return rc;
   b->tuned_ms = cand[best].ms;
+  return PM_OK;
+}
+
+int pm_batch_set_proposal(pm_batch* b, pm_net* net) {
+  if (!b) return PM_ERR_ARG;
+  PB_HIP(b, hipSetDevice(b->prm.device));
+  if (net) {
+    if (pm::net::device_of(net) != b->prm.device)
+      return fail(b, PM_ERR_ARG, "the net is on device %d, the batch on %d",
+                  pm::net::device_of(net), b->prm.device);
+    int rc;
+    if ((rc = reserve_net(b, net))) return rc;
+  }
+  if (net != b->net) {
+    b->net = net;
+    drop_graph(b);
+    b->ran = false;
+  }
   return PM_OK;
 }
 

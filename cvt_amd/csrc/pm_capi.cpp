@@ -875,20 +875,19 @@ int pm_setup_proposal(pm_ctx* c, pm_net* net) {
   const float* img[2] = {static_cast<const float*>(c->img[0].ptr),
                          static_cast<const float*>(c->img[1].ptr)};
   const int W = c->geo.W, H = c->geo.H, C = c->channels;
-  for (int view = 0; view < 2; ++view) {
-    const float* prop = nullptr;
-    int Wp = 0, os = 0;
-    // the right view's proposal: the network on the mirrored pair
-    if (pm::net::proposal(net, c->stream, img[0], img[1], W, H, C, W * C,
-                          view, &prop, &Wp, &os))
-      return fail(c, PM_ERR_HIP, "proposal network: %s",
-                  pm::net::error_of(net));
+  // the network on the pair (left view) and the mirrored pair (right view)
+  const float* prop[2] = {nullptr, nullptr};
+  int Wp = 0, os = 0;
+  if (pm::net::proposal(net, c->stream, img[0], img[1], W, H, C, W * C, prop,
+                        &Wp, &os))
+    return fail(c, PM_ERR_HIP, "proposal network: %s",
+                pm::net::error_of(net));
+  for (int view = 0; view < 2; ++view)
     PM_HIP(c, pm::launch_init_proposal(
                   c->stream, c->geo, view_bufs(c, view),
                   view_bufs(c, 1 - view), c->prm.disparity_max,
-                  view == 0 ? 1 : 0, prop, os, Wp, PM_NET_DISP_SCALE,
+                  view == 0 ? 1 : 0, prop[view], os, Wp, PM_NET_DISP_SCALE,
                   state_ptr(c, view, 0)));
-  }
   PM_HIP(c, pm::launch_viewbuf_clear(
                 c->stream, c->geo,
                 static_cast<unsigned long long*>(c->vbuf[1].ptr)));

@@ -91,6 +91,7 @@ struct pm_net {
   // activations (bf16; OUT fp32) and their gradients (fp32) for the
   // largest N x H x W seen
   long cap_pix = 0;  // N * H * W the buffers hold
+  long generation = 0;  // count of reallocations of the buffers
   DevBuf act[kTensors], gact[kTensors];
   DevBuf tmp;        // input gradient of a concatenated input (fp32)
   DevBuf gz;         // masked output gradient (bf16)
@@ -160,6 +161,7 @@ int alloc_batch(pm_net* n, int N, int H, int W) {
     if ((rc = ensure(n, n->tmp, tmp * 4)) || (rc = ensure(n, n->gz, gz * 2)))
       return rc;
     n->cap_pix = pix;
+    ++n->generation;
   }
   n->N = N;
   n->H = H;
@@ -277,19 +279,37 @@ namespace pm {
 namespace net {
 
 int proposal(pm_net* n, hipStream_t st, const float* left, const float* right,
-             int W, int H, int channels, int stride, int mirror,
-             const float** out, int* Wp, int* os) {
+             int W, int H, int channels, int stride, const float** out,
+             int* Wp, int* os) {
   const int Hp = (H + 7) / 8 * 8, Wq = (W + 7) / 8 * 8;
   int rc;
-  if ((rc = alloc_batch(n, 1, Hp, Wq))) return rc;
-  NET_HIP(n, launch_pack_pair(st, left, right, W, H, channels, stride, Hp, Wq,
-                              mirror, as<bf16>(n->act[X0])));
+  if ((rc = alloc_batch(n, 2, Hp, Wq))) return rc;
+  // batch entry 0: the pair; entry 1: the mirrored pair.  Every output
+  // pixel depends on its own batch entry only, so each entry's result is
+  // what a pass over it alone gives, bit for bit.
+  const size_t plane = (size_t)Hp * Wq;
+  for (int mirror = 0; mirror < 2; ++mirror)
+    NET_HIP(n, launch_pack_pair(st, left, right, W, H, channels, stride, Hp,
+                                Wq, mirror,
+                                as<bf16>(n->act[X0]) +
+                                    mirror * plane * kT[X0].ch));
   if ((rc = forward(n, st))) return rc;
-  *out = as<float>(n->act[OUT]);
+  out[0] = as<float>(n->act[OUT]);
+  out[1] = out[0] + plane * kT[OUT].ch;
   *Wp = Wq;
   *os = kT[OUT].ch;
   return PM_OK;
 }
+
+int reserve(pm_net* n, int Hp, int Wp) {
+  if (Hp < 8 || Wp < 8 || Hp % 8 || Wp % 8)
+    return fail(n, PM_ERR_ARG, "reserve: %d x %d is not a positive multiple "
+                "of 8", Hp, Wp);
+  NET_HIP(n, hipSetDevice(n->device));
+  return alloc_batch(n, 2, Hp, Wp);
+}
+
+long generation(const pm_net* n) { return n->generation; }
 
 const char* error_of(const pm_net* n) { return n->err.c_str(); }
 int device_of(const pm_net* n) { return n->device; }

@@ -2,7 +2,8 @@
 weights file.
 
 Data: seeded synthetic 512 x 384 RGBA pairs from cvt_amd.synth_gpu
-(8-24 cells, disparity 0-128 px), generated on the GPU batch by batch.
+(8-24 cells, disparity 0-128 px, or a disparity range drawn per batch from
+--dmax), generated on the GPU batch by batch.
 Targets: the plane fields plain PatchMatch produces on each pair (this
 library's engine, 16 iterations, 9 x 9 patch, the run's cost), LR-checked
 at 1 px: disparity / 128 and the plane's normal where the check keeps the
@@ -39,30 +40,34 @@ class TargetMaker:
     """Plain-PatchMatch targets of a batch of pairs on `contexts` engine
     contexts whose streams run side by side."""
 
-    def __init__(self, W, H, iters, cost_kind, cost_weight, contexts=2):
+    def __init__(self, W, H, iters, cost_kind, cost_weight, contexts=2,
+                 dmax=(128.0,)):
         import torch
 
         import cvt_amd
         self.W, self.H, self.iters = W, H, iters
         self.streams = [torch.cuda.Stream() for _ in range(contexts)]
-        self.ctx = [cvt_amd.Context(radius=4, disparity_max=128.0,
-                                    cost_kind=cost_kind,
-                                    cost_weight=cost_weight,
-                                    stream=s.cuda_stream)
-                    for s in self.streams]
-        self.disp = [torch.empty((H, W), device="cuda") for _ in self.ctx]
+        # contexts per disparity range (a context parameter)
+        self.ctxs = {d: [cvt_amd.Context(radius=4, disparity_max=d,
+                                         cost_kind=cost_kind,
+                                         cost_weight=cost_weight,
+                                         stream=s.cuda_stream)
+                         for s in self.streams] for d in dmax}
+        self.disp = [torch.empty((H, W), device="cuda")
+                     for _ in self.streams]
 
-    def __call__(self, left, right):
+    def __call__(self, left, right, dmax=128.0):
         import torch
 
         import cvt_amd
         B = left.shape[0]
+        ctxs = self.ctxs[dmax]
         torch.cuda.synchronize()
         targets = torch.empty((B, self.H, self.W, 4), device="cuda")
-        for b0 in range(0, B, len(self.ctx)):
-            group = list(range(b0, min(B, b0 + len(self.ctx))))
+        for b0 in range(0, B, len(ctxs)):
+            group = list(range(b0, min(B, b0 + len(ctxs))))
             for k, b in enumerate(group):
-                c = self.ctx[k]
+                c = ctxs[k]
                 c.set_images_device(left[b].data_ptr(), right[b].data_ptr(),
                                     self.W, self.H, 4)
                 c.setup()
@@ -70,7 +75,7 @@ class TargetMaker:
                 c.disparity_device(self.disp[k].data_ptr(),
                                    cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 180.0)
             for k, b in enumerate(group):
-                planes = torch.from_numpy(self.ctx[k].planes(0)).cuda()
+                planes = torch.from_numpy(ctxs[k].planes(0)).cuda()
                 torch.cuda.synchronize()
                 d = self.disp[k]
                 t = torch.cat([(d / 128.0)[..., None], to_normal(planes)], -1)
@@ -80,8 +85,9 @@ class TargetMaker:
         return targets
 
     def close(self):
-        for c in self.ctx:
-            c.close()
+        for cs in self.ctxs.values():
+            for c in cs:
+                c.close()
 
 
 def main():
@@ -94,12 +100,19 @@ def main():
                     help="PatchMatch iterations of the targets")
     ap.add_argument("--cost", choices=["color_grad", "census"],
                     default="color_grad")
+    ap.add_argument("--dmax", default="128",
+                    help="disparity range of the pairs, or a comma list "
+                         "to draw one per batch from")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--init", default=None, help="start from these weights")
     ap.add_argument("--out", default=None)
     ap.add_argument("--time-limit", type=float, default=0.0,
-                    help="stop after this many seconds (0: no limit)")
+                    help="stop after this many seconds (0: no limit); the "
+                         "learning-rate decay then follows whichever of "
+                         "the pair count and the time runs out first")
+    ap.add_argument("--save-every", type=int, default=0,
+                    help="also write the weights every this many steps")
     args = ap.parse_args()
 
     import torch
@@ -116,7 +129,9 @@ def main():
     net = ProposalNet(seed=args.seed, stream=stream.cuda_stream)
     if args.init:
         net.load(args.init)
-    maker = TargetMaker(W, H, args.iters, cost_kind, cost_weight)
+    dmaxes = tuple(float(v) for v in args.dmax.split(","))
+    maker = TargetMaker(W, H, args.iters, cost_kind, cost_weight,
+                        dmax=dmaxes)
     steps = (args.pairs + B - 1) // B
     rng = np.random.default_rng(args.seed)
     t0 = time.time()
@@ -125,13 +140,17 @@ def main():
     for step in range(steps):
         seed = args.seed * 1000003 + step
         cells = int(rng.integers(8, 25))
+        dmax = dmaxes[int(rng.integers(0, len(dmaxes)))]
         left, right, _ = synth_gpu.make_pairs(B, W, H, cells=cells,
-                                              dmax=128.0, seed=seed)
-        target = maker(left, right)
+                                              dmax=dmax, seed=seed)
+        target = maker(left, right, dmax)
         x = torch.cat([left, right], -1).contiguous()
         torch.cuda.synchronize()
-        lr = args.lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * step /
-                                                          steps)))
+        frac = step / steps
+        if args.time_limit:
+            frac = max(frac, (time.time() - t0) / args.time_limit)
+        lr = args.lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi *
+                                                          min(frac, 1.0))))
         loss = net.train_step(x.data_ptr(), target.data_ptr(), B, H, W, lr)
         losses.append(loss)
         done += B
@@ -139,6 +158,8 @@ def main():
             log("step %d/%d pairs %d loss %.5f (mean of last 25 %.5f) lr %.2e "
                 "%.1f s" % (step, steps, done, loss, float(np.mean(losses[-25:])),
                             lr, time.time() - t0))
+        if args.save_every and (step + 1) % args.save_every == 0:
+            net.save(args.out or WEIGHTS)
         if args.time_limit and time.time() - t0 > args.time_limit:
             log("time limit reached after %d pairs" % done)
             break

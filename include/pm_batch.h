@@ -26,6 +26,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "pm_net.h"
 #include "pm_stereo.h"
 
 #ifdef __cplusplus
@@ -104,6 +105,16 @@ int pm_batch_autotune(pm_batch* b, int reps, pm_batch_tuning* table,
                       int table_size, int* count);
 int pm_batch_get_tuning(const pm_batch* b, pm_batch_tuning* out);
 int pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t);
+
+/* Seed every pair from the plane-proposal network (include/pm_net.h)
+ * instead of the random init: each run's setup does, per pair, what
+ * pm_setup_proposal does for a context (the network on the pair and on
+ * the mirrored pair, each pixel keeping the proposed plane unless the
+ * random init plane costs less), inside the recorded graph.  The net must
+ * be on the batch's device and stay alive while the batch uses it; its
+ * activation buffers are sized here for the largest pair.  net = NULL
+ * restores the random init. */
+int pm_batch_set_proposal(pm_batch* b, pm_net* net);
 
 int pm_batch_sync(pm_batch* b);
 

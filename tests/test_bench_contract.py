@@ -37,3 +37,16 @@ def test_traffic_is_reported_only_for_the_profiled_sources(tmp_path):
     assert bench.profiled_traffic(*args) is None
     assert bench.profiled_traffic(str(tmp_path / "none.json"), (str(src),),
                                   "hbm_bytes_per_launch") is None
+
+
+def test_matched_iterations_is_the_fewest_that_reach_the_target():
+    sys.path.insert(0, ROOT)
+    import bench
+    # bad-1px after k = 0, 1, 2, ... refine iterations
+    bad = [0.9, 0.30, 0.16, 0.14, 0.15]
+    assert bench.matched_iterations(bad, 0.16) == 2
+    assert bench.matched_iterations(bad, 0.145) == 3
+    assert bench.matched_iterations(bad, 0.30) == 1
+    assert bench.matched_iterations(bad, 0.10) is None
+    # k = 0 (the proposal alone) never counts as a refine count
+    assert bench.matched_iterations([0.05, 0.2], 0.1) is None

@@ -185,3 +185,33 @@ def test_set_pair_swaps_images_without_recording(inputs, synth):
         assert same(b.disparity(0), want[2])
         with pytest.raises(cvt_amd.PMError):
             b.set_pair(0, dict(batch_pairs([q2])[0], width=q0["W"] + 1))
+
+
+def test_proposal_seeded_batch_equals_single_pairs(inputs):
+    """pm_batch_set_proposal: every pair seeded as pm_setup_proposal seeds a
+    context (an untrained network: the seeding, not its quality, is under
+    test), then two iterations, inside the recorded graph"""
+    from cvt_amd.net import ProposalNet
+    kind, weight = COSTS[1]
+    with ProposalNet(seed=5) as net:
+        want = []
+        for q in inputs:
+            with cvt_amd.Context(radius=RADIUS, disparity_max=q["dmax"],
+                                 cost_kind=kind, cost_weight=weight) as ctx:
+                ctx.set_images(q["L"], q["R"])
+                ctx.setup_proposal(net)
+                ctx.iterate(2)
+                want.append((ctx.planes(0), ctx.planes(1),
+                             ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0,
+                                           5.0)))
+        with Batch(batch_pairs(inputs), radius=RADIUS, cost_kind=kind,
+                   cost_weight=weight) as b:
+            b.set_proposal(net)
+            for replay in range(2):
+                b.run(2, cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)
+                check_batch(b, inputs, want, ("proposal", replay))
+            # back to the random init
+            b.set_proposal(None)
+            b.run(1)
+            rnd = single(inputs[0], kind, weight, 1)
+            assert same(b.planes(0, 0), rnd[0])
