@@ -117,6 +117,8 @@ SIGNATURES = {
     "pm_eval_error_device": (_i, [_p, _i, _p, _i, _i, _i, _f, _f, _p, _p]),
     "pm_error_histogram": (_i, [_p, _i, _p, _i, _i, _i, _i, _p, _i, _p]),
     "pm_default_histogram": (_i, [_i, _p, _p]),
+    "pm_disparity_planes_noalloc": (_i, [_p, _p, _i, _i, _i, _f, _f, _p, _i,
+                                         _p]),
 }
 
 _lib = None

@@ -19,7 +19,8 @@ ROOT = os.path.dirname(HERE)
 SO = os.path.join(HERE, "libcvt_amd_pm.so")
 SOURCES = ["csrc/pm_engine.hip", "csrc/pm_smooth.hip", "csrc/pm_net.hip",
            "csrc/pm_capi.cpp", "csrc/pm_net_capi.cpp",
-           "csrc/pm_batch.cpp", "csrc/PMHuberStereo.cpp", "csrc/pm_io.cpp"]
+           "csrc/pm_batch.cpp", "csrc/PMHuberStereo.cpp", "csrc/pm_io.cpp",
+           "csrc/pm_noalloc.hip"]
 ARCH = os.environ.get("PM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -246,6 +246,9 @@ size_t batch_lds_bytes(int R, int cost_kind, int rows);
 hipError_t batch_resident_per_cu(int cost_kind, int rows, size_t lds,
                                  int* out);
 
+// pm_io_last_error's message on this thread (pm_io.cpp); returns code
+int io_set_error(int code, const char* msg);
+
 hipError_t launch_draws(hipStream_t st, uint64_t k, int n, float* out);
 
 hipError_t launch_math(hipStream_t st, const float* in, int n, float* out);

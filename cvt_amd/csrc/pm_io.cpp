@@ -28,6 +28,19 @@ int io_fail(int code, const char* fmt, ...) {
   return code;
 }
 
+}  // namespace
+
+namespace pm {
+// the message pm_io_last_error returns on this thread, for the library's
+// other context-free entry points
+int io_set_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace pm
+
+namespace {
+
 bool host_little_endian() {
   const uint32_t one = 1;
   unsigned char b;

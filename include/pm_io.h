@@ -71,6 +71,23 @@ int pm_eval_error_device(const float* d_disparity, int stride_d,
                          float abs_thres, float rel_thres,
                          uint64_t* d_counts, void* stream);
 
+/* pm_disparity_device on caller-owned plane fields, allocating nothing.
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops).
+ * d_left_planes, d_right_planes: W x H x 4 floats (a, b, c, cost) in HIP
+ * device memory of `device`, the two views' fields as pm_get_planes
+ * returns them; d_disparity: W x H floats.  pp, maxdispdiff and maxdeg as
+ * pm_disparity.  Stream-ordered on `stream` (a hipStream_t of `device`,
+ * NULL = its null stream), no host synchronisation.  The LR-checked field
+ * lives in a fixed array of 2^21 float4 per GPU (32 MiB, held until the
+ * process ends): W x H above 2^21 pixels (1920 x 1080 fits) is refused
+ * with PM_ERR_ARG.  Calls that overlap on one GPU share that array and
+ * race.  Failures leave their message in pm_io_last_error(). */
+int pm_disparity_planes_noalloc(const float* d_left_planes,
+                                const float* d_right_planes, int W, int H,
+                                int pp, float maxdispdiff, float maxdeg,
+                                float* d_disparity, int device, void* stream);
+
 /* Disparity::histogramErrorAbsolute / histogramErrorRelative
  * (Disparity.cpp:236-318): over pixels where both maps are finite and
  * > 0, the fraction whose error e = |test - gt| (relative: divided by
