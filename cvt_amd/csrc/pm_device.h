@@ -4,7 +4,12 @@
 // Arithmetic contract (the whole engine is built with -ffp-contract=off and
 // correctly rounded float32 division / sqrt): every expression below is
 // evaluated as written, left to right, one rounding per operation, except
-// the fused operations written as fmaf().  exp, sin/cos and acospi are the
+// the fused operations written as fmaf().  Those are the contractions an
+// OpenCL compiler with FP_CONTRACT ON forms in the reference's source
+// expressions: a product operand of a sum or difference fused into it,
+// the left one first (a * b + c, a + b * c, a - b * c, a -= b * c), and a
+// built-in dot as the chain fma(x2, y2, fma(x1, y1, x0 * y0)); never
+// across statements or helper calls.  exp, sin/cos and acospi are the
 // engine's own Cody-Waite + Taylor implementations, so every result is a
 // defined function of the inputs, independent of the vendor math library.
 #pragma once
@@ -193,7 +198,7 @@ __device__ __forceinline__ float4 one_minus(float4 s) {
 
 // OpenCL normalize(): v / |v|, zero vector unchanged
 __device__ __forceinline__ float3 normalize3(float3 v) {
-  float l2 = (v.x * v.x + v.y * v.y) + v.z * v.z;
+  float l2 = fmaf(v.z, v.z, fmaf(v.y, v.y, v.x * v.x));
   if (l2 == 0.0f) return v;
   float l = sqrtf(l2);
   return make_float3(v.x / l, v.y / l, v.z / l);
@@ -203,16 +208,17 @@ __device__ __forceinline__ float3 normalize3(float3 v) {
 __device__ __forceinline__ float3 to_normal(float4 s0) {
   float4 s = one_minus(s0);
   float3 n;
-  n.z = 1.0f / sqrtf((s.x * s.x + s.y * s.y) + 1.0f);
+  n.z = 1.0f / sqrtf(fmaf(s.x, s.x, s.y * s.y) + 1.0f);
   n.x = -s.x * n.z;
   n.y = -s.y * n.z;
-  n.z = sqrtf((1.0f - n.x * n.x) - n.y * n.y);
+  n.z = sqrtf(fmaf(-n.y, n.y, fmaf(-n.x, n.x, 1.0f)));
   return n;
 }
 
 __device__ __forceinline__ float4 plane_from_nd(float3 n, float x, float y,
                                                 float d) {
-  float4 t = mk4(-n.x / n.z, -n.y / n.z, (n.x * x + n.y * y) / n.z + d, 0.0f);
+  float4 t = mk4(-n.x / n.z, -n.y / n.z, fmaf(n.x, x, n.y * y) / n.z + d,
+                 0.0f);
   return mk4(1.0f - t.x, 0.0f - t.y, 0.0f - t.z, 0.0f - t.w);
 }
 
@@ -236,7 +242,7 @@ __device__ __forceinline__ float4 state_init(float u0, float u1, float u2,
   float z = u0 * dmax;
   float az = azimuth_of(u1);
   float u = u2 * kNormalCompMax;
-  float w = sqrtf(1.0f - u * u);
+  float w = sqrtf(fmaf(-u, u, 1.0f));
   float sn, cs;
   sincos_f(az, &sn, &cs);
   float3 n = make_float3(w * cs, w * sn, u);
@@ -255,16 +261,16 @@ __device__ __forceinline__ float4 state_refine(float u0, float u1, float u2,
   float4 st = s0;
   if (!lr) st = viewprop(st);
   st = one_minus(st);
-  float z = (st.x * x + st.y * y) + st.z;
+  float z = fmaf(st.x, x, st.y * y) + st.z;
   float3 n;
-  n.z = 1.0f / sqrtf((st.x * st.x + st.y * st.y) + 1.0f);
+  n.z = 1.0f / sqrtf(fmaf(st.x, st.x, st.y * st.y) + 1.0f);
   n.x = -st.x * n.z;
   n.y = -st.y * n.z;
-  z += ((u0 - 0.5f) * 2.0f) * kDepthRefineMul;
+  z = fmaf((u0 - 0.5f) * 2.0f, kDepthRefineMul, z);
   z = fminf(fmaxf(z, 0.0f), dmax);
   float az = azimuth_of(u1);
   float u = u2 * (1.0f - kNormalRefineMul);
-  float w = sqrtf(1.0f - u * u);
+  float w = sqrtf(fmaf(-u, u, 1.0f));
   float sn, cs;
   sincos_f(az, &sn, &cs);
   float3 n2 = make_float3(w * cs, w * sn, u);
@@ -287,7 +293,7 @@ __device__ __forceinline__ float4 to_ref_normal_depth(float4 s0, float x,
   st = one_minus(st);
   float4 r;
   r.w = (transform(s0, x, y) - x) * sidesign;
-  r.z = 1.0f / sqrtf((st.x * st.x + st.y * st.y) + 1.0f);
+  r.z = 1.0f / sqrtf(fmaf(st.x, st.x, st.y * st.y) + 1.0f);
   r.x = -st.x * r.z;
   r.y = -st.y * r.z;
   if (!(finite_f(r.x) && finite_f(r.y) && finite_f(r.z)))
@@ -303,7 +309,7 @@ __device__ __forceinline__ float smooth_dist(float4 s, float4 sm, float x,
   float den = id2 + 0.3f;
   float dg = 0.1f / den, dw = id2 / den;
   float dx = a.x - sm.x, dy = a.y - sm.y, dz = a.z - sm.z, dd = a.w - sm.w;
-  return ((dx * (dg * dx) + dy * (dg * dy)) + dz * (dg * dz)) + dd * (dw * dd);
+  return fmaf(dd, dw * dd, fmaf(dz, dg * dz, fmaf(dy, dg * dy, dx * (dg * dx))));
 }
 
 // gray value used by the support weight: the built-in
@@ -359,7 +365,7 @@ __device__ __forceinline__ float gray_at(const float* __restrict__ img, int W,
                                          int y) {
   if (x < 0 || x >= W || y < 0 || y >= H) return 0.0f;  // CLK_ADDRESS_CLAMP
   const float4 v = fetch_rgba(img, C, stride, x, y);
-  return ((v.x * 0.2126f + v.y * 0.7152f) + v.z * 0.0722f) + v.w * 0.0f;
+  return fmaf(v.w, 0.0f, fmaf(v.z, 0.0722f, fmaf(v.y, 0.7152f, v.x * 0.2126f)));
 }
 
 __device__ __forceinline__ uint32_t signmask4(float4 g) {
@@ -369,7 +375,7 @@ __device__ __forceinline__ uint32_t signmask4(float4 g) {
 }
 
 __device__ __forceinline__ float len3(float4 s) {
-  return sqrtf((s.x * s.x + s.y * s.y) + s.z * s.z);
+  return sqrtf(fmaf(s.z, s.z, fmaf(s.y, s.y, s.x * s.x)));
 }
 
 }  // namespace pm

@@ -512,18 +512,20 @@ __device__ float4 klt_pass(const Lane& L, float4 s) {
   float m20 = H02, m21 = H12, m22 = H22;
   m10 /= m00;
   m20 /= m00;
-  m11 -= m10 * m01;
-  m12 -= m10 * m02;
-  m21 -= m20 * m01;
-  m22 -= m20 * m02;
+  // x -= a * b contracts to fma(-a, b, x); the built-in dot of two
+  // components is fma(y0, y1, x0 * x1)
+  m11 = fmaf(-m10, m01, m11);
+  m12 = fmaf(-m10, m02, m12);
+  m21 = fmaf(-m20, m01, m21);
+  m22 = fmaf(-m20, m02, m22);
   m21 /= m11;
-  m22 -= m21 * m12;
+  m22 = fmaf(-m21, m12, m22);
   float b0 = J0, b1 = J1, b2 = J2;
-  b1 -= m10 * b0;
-  b2 -= m20 * b0 + m21 * b1;
+  b1 = fmaf(-m10, b0, b1);
+  b2 -= fmaf(m21, b1, m20 * b0);
   const float r2 = b2 / m22;
-  const float r1 = (b1 - m12 * r2) / m11;
-  const float r0 = (b0 - (m01 * r1 + m02 * r2)) / m00;
+  const float r1 = fmaf(-m12, r2, b1) / m11;
+  const float r0 = (b0 - fmaf(m02, r2, m01 * r1)) / m00;
   return mk4(s.x - r0, s.y - r1, s.z - r2, s.w - 1e5f);
 }
 
@@ -739,12 +741,12 @@ __device__ __forceinline__ void try_candidate(const Lane& L, const PropCtx& c,
   if (active) {
     ds = smooth_dist(self, c.smooth, L.xf, L.yf, c.dmax, c.lr);
     dc = smooth_dist(cand, c.smooth, L.xf, L.yf, c.dmax, c.lr);
-    mincost = (self.w + c.theta * ds) - c.theta * dc;
+    mincost = fmaf(-c.theta, dc, fmaf(c.theta, ds, self.w));
   }
   const float w = cost_any<kKind, kCount>(L, cand, mincost, active, ntaps);
   if (active) {
     cand.w = w;
-    if (cand.w + c.theta * dc <= self.w + c.theta * ds) self = cand;
+    if (fmaf(c.theta, dc, cand.w) <= fmaf(c.theta, ds, self.w)) self = cand;
   }
 }
 
@@ -832,8 +834,8 @@ __device__ __forceinline__ void propagate_block(
   for (int i = 0; i < kNumRndSample; ++i) {
     const float ux = mwc_next(&rng);
     const float uy = mwc_next(&rng);
-    const int ox = (int)(((ux - 0.5f) * 2.0f) * kRndSampleRadius + 0.5f);
-    const int oy = (int)(((uy - 0.5f) * 2.0f) * kRndSampleRadius + 0.5f);
+    const int ox = (int)fmaf((ux - 0.5f) * 2.0f, kRndSampleRadius, 0.5f);
+    const int oy = (int)fmaf((uy - 0.5f) * 2.0f, kRndSampleRadius, 0.5f);
     float4 cand = mk4(0.0f, 0.0f, 0.0f, 0.0f);
     if (L.valid) {
       const int nx = clampi(L.x + ox, 0, W - 1), ny = clampi(L.y + oy, 0, H - 1);
@@ -962,8 +964,8 @@ __device__ __forceinline__ void consistency_pixel(
     const float3 nl = to_normal(sl);
     const float3 n1 = to_normal(viewprop(r1));
     const float3 n2 = to_normal(viewprop(r2));
-    const float nd1 = (nl.x * n1.x + nl.y * n1.y) + nl.z * n1.z;
-    const float nd2 = (nl.x * n2.x + nl.y * n2.y) + nl.z * n2.z;
+    const float nd1 = fmaf(nl.z, n1.z, fmaf(nl.y, n1.y, nl.x * n1.x));
+    const float nd2 = fmaf(nl.z, n2.z, fmaf(nl.y, n2.y, nl.x * n2.x));
     const float dm1 = fabsf(xf - transform(r1, x2, yf));
     const float dm2 = fabsf(xf - transform(r2, x2, yf));
     const float a1 = acospi_f(fminf(fmaxf(nd1, -1.0f), 1.0f));

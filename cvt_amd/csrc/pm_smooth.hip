@@ -35,15 +35,15 @@ k_diff_tensor(const float* __restrict__ img, int W, int H, int C, int stride,
   const float g = dt_gray(img, W, H, C, stride, x, y);
   float dx = dt_gray(img, W, H, C, stride, x + 1, y) - g;
   float dy = dt_gray(img, W, H, C, stride, x, y + 1) - g;
-  const float len = sqrtf(dx * dx + dy * dy);
+  const float len = sqrtf(fmaf(dx, dx, dy * dy));
   const float w = exp_nonpos(-alpha * len);
   float4 o = mk4(1.0f, 0.0f, 0.0f, 1.0f);
   if (!(len <= 0.001f)) {
     dx = dx / len;
     dy = dy / len;
     const float dxo = dy, dyo = -dx;
-    o = mk4((w * dx) * dx + dxo * dxo, (w * dx) * dy + dxo * dyo,
-            (w * dy) * dx + dyo * dxo, (w * dy) * dy + dyo * dyo);
+    o = mk4(fmaf(w * dx, dx, dxo * dxo), fmaf(w * dx, dy, dxo * dyo),
+            fmaf(w * dy, dx, dyo * dxo), fmaf(w * dy, dy, dyo * dyo));
   }
   dt[(size_t)y * W + x] = o;
 }
@@ -142,15 +142,15 @@ __device__ __forceinline__ Dual dual_of(const float4& u0, const float4& ux,
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const float dx = comp(u0, k) - comp(ux, k), dy = comp(u0, k) - comp(uy, k);
-    const float DTdx = D.x * dx + D.z * dy;
-    const float DTdy = D.y * dx + D.w * dy;
-    r.lo[k] = comp(plo4, k) + kRofSigma * DTdx;
-    r.hi[k] = comp(phi4, k) + kRofSigma * DTdy;
+    const float DTdx = fmaf(D.x, dx, D.z * dy);
+    const float DTdy = fmaf(D.y, dx, D.w * dy);
+    r.lo[k] = fmaf(kRofSigma, DTdx, comp(plo4, k));
+    r.hi[k] = fmaf(kRofSigma, DTdy, comp(phi4, k));
     if (kHuber) {
       r.lo[k] = r.lo[k] / shrink;
       r.hi[k] = r.hi[k] / shrink;
     }
-    n[k] = r.lo[k] * r.lo[k] + r.hi[k] * r.hi[k];
+    n[k] = fmaf(r.lo[k], r.lo[k], r.hi[k] * r.hi[k]);
   }
   const float pxyz =
       kHuber ? fmaxf(1.0f, sqrtf(((n[0] + n[1]) + n[2]) + n[3]))
@@ -165,8 +165,8 @@ __device__ __forceinline__ Dual dual_of(const float4& u0, const float4& ux,
       lo = lo / pr;
       hi = hi / pr;
     }
-    r.lo[k] = D.x * lo + D.z * hi;
-    r.hi[k] = D.y * lo + D.w * hi;
+    r.lo[k] = fmaf(D.x, lo, D.z * hi);
+    r.hi[k] = fmaf(D.y, lo, D.w * hi);
   }
   return r;
 }
@@ -184,11 +184,12 @@ __device__ __forceinline__ void rof_primal(const Dual& P, const Dual& Lf,
     const float lam = comp(lambda, k);
     const float div = ((P.lo[k] - Lf.lo[k]) + P.hi[k]) - Up.hi[k];
     const float img = comp(fi, k), qq = comp(qi, k), u0 = comp(ui, k);
-    float v = (u0 + kRofTau * ((lam * img - div) - qq)) /
-              (1.0f + kRofTau * lam);
-    v = v + kRofTheta * (v - u0);
-    float w = (qq + kRofEta * (v - img)) / (1.0f + 4.0f * kRofEta / lam);
-    w = w + kRofTheta * (w - qq);
+    float v = fmaf(kRofTau, fmaf(lam, img, -div) - qq, u0) /
+              fmaf(kRofTau, lam, 1.0f);
+    v = fmaf(kRofTheta, v - u0, v);
+    // 1 + 4 ETA / lambda: a quotient, nothing to fuse
+    float w = fmaf(kRofEta, v - img, qq) / (1.0f + 4.0f * kRofEta / lam);
+    w = fmaf(kRofTheta, w - qq, w);
     un[k] = v;
     qn[k] = w;
   }
@@ -227,7 +228,7 @@ k_rof_steps(int W, int H, int nsteps, float eps, const float4* __restrict__ u,
             float4* __restrict__ p_out, float4* __restrict__ q_out) {
   __shared__ float4 su[kRofRH * kRofRW];
   __shared__ float4 sp[2 * kRofRH * kRofRW];
-  const float shrink = 1.0f + kRofSigma * eps;  // Huber dual only
+  const float shrink = fmaf(kRofSigma, eps, 1.0f);  // Huber dual only
   const int lx = threadIdx.x;
   const int rx0 = blockIdx.x * kRofTX - kRofK;
   const int ry0 = blockIdx.y * kRofTY - kRofK;

@@ -282,7 +282,7 @@ static inline f4 one_minus(f4 s)
 /* OpenCL normalize(): v / |v|, zero vector returned unchanged */
 static f3 normalize3(f3 v)
 {
-    float l2 = (v.x * v.x + v.y * v.y) + v.z * v.z;
+    float l2 = fmaf(v.z, v.z, fmaf(v.y, v.y, v.x * v.x));
     if (l2 == 0.0f) return v;
     float l = sqrtf(l2);
     f3 r = {v.x / l, v.y / l, v.z / l};
@@ -294,10 +294,10 @@ static f3 to_normal(f4 s0)
 {
     f4 s = one_minus(s0);
     f3 n;
-    n.z = 1.0f / sqrtf((s.x * s.x + s.y * s.y) + 1.0f);
+    n.z = 1.0f / sqrtf(fmaf(s.x, s.x, s.y * s.y) + 1.0f);
     n.x = -s.x * n.z;
     n.y = -s.y * n.z;
-    n.z = sqrtf((1.0f - n.x * n.x) - n.y * n.y);
+    n.z = sqrtf(fmaf(-n.y, n.y, fmaf(-n.x, n.x, 1.0f)));
     return n;
 }
 
@@ -305,7 +305,8 @@ static f3 to_normal(f4 s0)
  * (1,0,0,0) - (-nx/nz, -ny/nz, (nx x + ny y)/nz + d, 0) */
 static f4 plane_from_nd(f3 n, float x, float y, float d)
 {
-    f4 t = mk4(-n.x / n.z, -n.y / n.z, (n.x * x + n.y * y) / n.z + d, 0.0f);
+    f4 t = mk4(-n.x / n.z, -n.y / n.z, fmaf(n.x, x, n.y * y) / n.z + d,
+               0.0f);
     return mk4(1.0f - t.x, 0.0f - t.y, 0.0f - t.z, 0.0f - t.w);
 }
 
@@ -331,7 +332,7 @@ static f4 state_init(float u0, float u1, float u2, float x, float y, int lr,
     float z = u0 * dmax;
     float az = azimuth_of(u1);
     float u = u2 * NORMALCOMPMAX;
-    float w = sqrtf(1.0f - u * u);
+    float w = sqrtf(fmaf(-u, u, 1.0f));
     float sn, cs;
     pmo_sincosf(az, &sn, &cs);
     f3 n = {w * cs, w * sn, u};
@@ -351,16 +352,16 @@ static f4 state_refine(float u0, float u1, float u2, f4 s0, float x, float y,
     f4 st = s0;
     if (!lr) st = viewprop(st);
     st = one_minus(st);
-    float z = (st.x * x + st.y * y) + st.z;
+    float z = fmaf(st.x, x, st.y * y) + st.z;
     f3 n;
-    n.z = 1.0f / sqrtf((st.x * st.x + st.y * st.y) + 1.0f);
+    n.z = 1.0f / sqrtf(fmaf(st.x, st.x, st.y * st.y) + 1.0f);
     n.x = -st.x * n.z;
     n.y = -st.y * n.z;
-    z += ((u0 - 0.5f) * 2.0f) * DEPTHREFINEMUL;
+    z = fmaf((u0 - 0.5f) * 2.0f, DEPTHREFINEMUL, z);
     z = fminf(fmaxf(z, 0.0f), dmax);
     float az = azimuth_of(u1);
     float u = u2 * (1.0f - NORMALREFINEMUL);
-    float w = sqrtf(1.0f - u * u);
+    float w = sqrtf(fmaf(-u, u, 1.0f));
     float sn, cs;
     pmo_sincosf(az, &sn, &cs);
     f3 n2 = {w * cs, w * sn, u};
@@ -383,7 +384,7 @@ static f4 to_ref_normal_depth(f4 s0, float x, float y, int lr)
     st = one_minus(st);
     f4 r;
     r.w = (transform(s0, x, y) - x) * sidesign;
-    r.z = 1.0f / sqrtf((st.x * st.x + st.y * st.y) + 1.0f);
+    r.z = 1.0f / sqrtf(fmaf(st.x, st.x, st.y * st.y) + 1.0f);
     r.x = -st.x * r.z;
     r.y = -st.y * r.z;
     if (!(isfinite(r.x) && isfinite(r.y) && isfinite(r.z)))
@@ -401,8 +402,9 @@ static float smooth_dist(f4 s, f4 smooth, float x, float y, float dmax,
     f4 diag = mk4(0.1f / den, 0.1f / den, 0.1f / den, id2 / den);
     f4 d = mk4(a.x - smooth.x, a.y - smooth.y, a.z - smooth.z,
                a.w - smooth.w);
-    return ((d.x * (diag.x * d.x) + d.y * (diag.y * d.y)) +
-            d.z * (diag.z * d.z)) + d.w * (diag.w * d.w);
+    return fmaf(d.w, diag.w * d.w,
+                fmaf(d.z, diag.z * d.z,
+                     fmaf(d.y, diag.y * d.y, d.x * (diag.x * d.x))));
 }
 
 void pmo_viewprop(const float* s, float* out)
@@ -493,8 +495,8 @@ static inline float gray_at(const float* img, int W, int H, int channels,
     if (x < 0 || x >= W || y < 0 || y >= H) return 0.0f;
     float v[4];
     fetch_rgba(img, channels, stride, x, y, v);
-    return ((v[0] * 0.2126f + v[1] * 0.7152f) + v[2] * 0.0722f) +
-           v[3] * 0.0f;
+    return fmaf(v[3], 0.0f, fmaf(v[2], 0.0722f,
+                                 fmaf(v[1], 0.7152f, v[0] * 0.2126f)));
 }
 
 /* pmhstereo_bilateral_weight_to_alpha (pmhstereo.cl:1462-1498) and
@@ -713,17 +715,19 @@ static void lusolve3(const float Hm[9], const float b_in[3], float out[3])
     float b0 = b_in[0], b1 = b_in[1], b2 = b_in[2];
     m[3] /= m[0];
     m[6] /= m[0];
-    m[4] -= m[3] * m[1];
-    m[5] -= m[3] * m[2];
-    m[7] -= m[6] * m[1];
-    m[8] -= m[6] * m[2];
+    /* x -= a * b contracts to fma(-a, b, x); the built-in dot of two
+     * components is fma(y0, y1, x0 * x1) */
+    m[4] = fmaf(-m[3], m[1], m[4]);
+    m[5] = fmaf(-m[3], m[2], m[5]);
+    m[7] = fmaf(-m[6], m[1], m[7]);
+    m[8] = fmaf(-m[6], m[2], m[8]);
     m[7] /= m[4];
-    m[8] -= m[7] * m[5];
-    b1 -= m[3] * b0;
-    b2 -= m[6] * b0 + m[7] * b1;
+    m[8] = fmaf(-m[7], m[5], m[8]);
+    b1 = fmaf(-m[3], b0, b1);
+    b2 -= fmaf(m[7], b1, m[6] * b0);
     out[2] = b2 / m[8];
-    out[1] = (b1 - m[5] * out[2]) / m[4];
-    out[0] = (b0 - (m[1] * out[1] + m[2] * out[2])) / m[0];
+    out[1] = fmaf(-m[5], out[2], b1) / m[4];
+    out[0] = (b0 - fmaf(m[2], out[2], m[1] * out[1])) / m[0];
 }
 
 static f4 klt_at(const float* tex1, const float* tex2, int W, int H, int R,
@@ -923,10 +927,11 @@ static void try_candidate(const prop_ctx* c, f4* self, f4 cand)
 {
     float ds = smooth_dist(*self, c->smooth, c->xf, c->yf, c->dmax, c->lr);
     float dc = smooth_dist(cand, c->smooth, c->xf, c->yf, c->dmax, c->lr);
-    float mincost = (self->w + c->theta * ds) - c->theta * dc;
+    float mincost = fmaf(-c->theta, dc, fmaf(c->theta, ds, self->w));
     cand.w = cost_at(c->tex1, c->tex2, c->W, c->H, c->R, c->m, c->x, c->y,
                      cand, mincost);
-    if (cand.w + c->theta * dc <= self->w + c->theta * ds) *self = cand;
+    if (fmaf(c->theta, dc, cand.w) <= fmaf(c->theta, ds, self->w))
+        *self = cand;
 }
 
 void pmo_propagate(const float* tex1, const float* tex2, int W, int H,
@@ -965,10 +970,10 @@ void pmo_propagate(const float* tex1, const float* tex2, int W, int H,
             for (int i = 0; i < NUMRNDSAMPLE; i++) {
                 float ux = mwc_next(&rng);
                 float uy = mwc_next(&rng);
-                int ox = (int)(((ux - 0.5f) * 2.0f) * NUMRNDSAMPLERADIUS +
-                               0.5f);
-                int oy = (int)(((uy - 0.5f) * 2.0f) * NUMRNDSAMPLERADIUS +
-                               0.5f);
+                int ox = (int)fmaf((ux - 0.5f) * 2.0f, NUMRNDSAMPLERADIUS,
+                                   0.5f);
+                int oy = (int)fmaf((uy - 0.5f) * 2.0f, NUMRNDSAMPLERADIUS,
+                                   0.5f);
                 int nx = x + ox, ny = y + oy;
                 nx = nx < 0 ? 0 : (nx > W - 1 ? W - 1 : nx);
                 ny = ny < 0 ? 0 : (ny > H - 1 ? H - 1 : ny);
@@ -1046,8 +1051,8 @@ void pmo_consistency(const float* left, const float* right, int W, int H,
                 f3 nl = to_normal(sl);
                 f3 n1 = to_normal(viewprop(r1));
                 f3 n2 = to_normal(viewprop(r2));
-                float nd1 = (nl.x * n1.x + nl.y * n1.y) + nl.z * n1.z;
-                float nd2 = (nl.x * n2.x + nl.y * n2.y) + nl.z * n2.z;
+                float nd1 = fmaf(nl.z, n1.z, fmaf(nl.y, n1.y, nl.x * n1.x));
+                float nd2 = fmaf(nl.z, n2.z, fmaf(nl.y, n2.y, nl.x * n2.x));
                 float dm1 = fabsf(xf - transform(r1, x2, yf));
                 float dm2 = fabsf(xf - transform(r2, x2, yf));
                 float a1 = pmo_acospif(fminf(fmaxf(nd1, -1.0f), 1.0f));
@@ -1070,7 +1075,7 @@ void pmo_to_disparity(const float* states, int W, int H, float scale, int lr,
             f4 s = load4(states, p);
             float d = (transform(s, (float)x, (float)y) - (float)x) *
                       (lr ? -1.0f : 1.0f);
-            float len = sqrtf((s.x * s.x + s.y * s.y) + s.z * s.z);
+            float len = sqrtf(fmaf(s.z, s.z, fmaf(s.y, s.y, s.x * s.x)));
             float v;
             if (len < 1e-4f || d < 0.0f || d > (float)W)
                 v = 0.0f;
@@ -1082,7 +1087,7 @@ void pmo_to_disparity(const float* states, int W, int H, float scale, int lr,
 
 static inline float len3(f4 s)
 {
-    return sqrtf((s.x * s.x + s.y * s.y) + s.z * s.z);
+    return sqrtf(fmaf(s.z, s.z, fmaf(s.y, s.y, s.x * s.x)));
 }
 
 /* nearest usable state left and right of x on row y (length(xyz) >= 0.1,
@@ -1196,7 +1201,7 @@ void pmo_diffusion_tensor(const float* img, int W, int H, int channels,
             float g = dt_gray(img, W, H, channels, stride, x, y);
             float dx = dt_gray(img, W, H, channels, stride, x + 1, y) - g;
             float dy = dt_gray(img, W, H, channels, stride, x, y + 1) - g;
-            float len = sqrtf(dx * dx + dy * dy);
+            float len = sqrtf(fmaf(dx, dx, dy * dy));
             float w = pmo_expf(-alpha * len);
             float* o = dt + ((size_t)y * W + x) * 4;
             if (len <= 0.001f) {
@@ -1208,10 +1213,10 @@ void pmo_diffusion_tensor(const float* img, int W, int H, int channels,
                 dx = dx / len;
                 dy = dy / len;
                 float dxo = dy, dyo = -dx;
-                o[0] = (w * dx) * dx + dxo * dxo;
-                o[1] = (w * dx) * dy + dxo * dyo;
-                o[2] = (w * dy) * dx + dyo * dxo;
-                o[3] = (w * dy) * dy + dyo * dyo;
+                o[0] = fmaf(w * dx, dx, dxo * dxo);
+                o[1] = fmaf(w * dx, dy, dxo * dyo);
+                o[2] = fmaf(w * dy, dx, dyo * dxo);
+                o[3] = fmaf(w * dy, dy, dyo * dyo);
             }
         }
 }
@@ -1319,18 +1324,18 @@ static void pd_dual(const float* u, const float* p, const float* dt, int W,
         memcpy(phi, p + q * 8 + 4, sizeof phi);
     }
     float n[4];
-    const float shrink = 1.0f + ROF_SIGMA * eps;
+    const float shrink = fmaf(ROF_SIGMA, eps, 1.0f);
     for (int k = 0; k < 4; k++) {
         float dx = u0[k] - ux[k], dy = u0[k] - uy[k];
-        float DTdx = D[0] * dx + D[2] * dy;
-        float DTdy = D[1] * dx + D[3] * dy;
-        plo[k] = plo[k] + ROF_SIGMA * DTdx;
-        phi[k] = phi[k] + ROF_SIGMA * DTdy;
+        float DTdx = fmaf(D[0], dx, D[2] * dy);
+        float DTdy = fmaf(D[1], dx, D[3] * dy);
+        plo[k] = fmaf(ROF_SIGMA, DTdx, plo[k]);
+        phi[k] = fmaf(ROF_SIGMA, DTdy, phi[k]);
         if (eps >= 0.0f) {
             plo[k] = plo[k] / shrink;
             phi[k] = phi[k] / shrink;
         }
-        n[k] = plo[k] * plo[k] + phi[k] * phi[k];
+        n[k] = fmaf(plo[k], plo[k], phi[k] * phi[k]);
     }
     float proj[4];
     if (eps >= 0.0f) {
@@ -1344,8 +1349,8 @@ static void pd_dual(const float* u, const float* p, const float* dt, int W,
     for (int k = 0; k < 4; k++) {
         plo[k] = plo[k] / proj[k];
         phi[k] = phi[k] / proj[k];
-        lo[k] = D[0] * plo[k] + D[2] * phi[k];
-        hi[k] = D[1] * plo[k] + D[3] * phi[k];
+        lo[k] = fmaf(D[0], plo[k], D[2] * phi[k]);
+        hi[k] = fmaf(D[1], plo[k], D[3] * phi[k]);
     }
 }
 
@@ -1368,12 +1373,14 @@ static void pd_step(const float* u, const float* p, const float* q,
                 float div = ((Plo[k] - Llo[k]) + Phi[k]) - Uhi[k];
                 float img = f[o * 4 + k], qq = q[o * 4 + k];
                 float uo = u[o * 4 + k];
-                float un = (uo + ROF_TAU * ((lambda[k] * img - div) - qq)) /
-                           (1.0f + ROF_TAU * lambda[k]);
-                un = un + ROF_THETA * (un - uo);
-                float qn = (qq + ROF_ETA * (un - img)) /
+                float un = fmaf(ROF_TAU,
+                                fmaf(lambda[k], img, -div) - qq, uo) /
+                           fmaf(ROF_TAU, lambda[k], 1.0f);
+                un = fmaf(ROF_THETA, un - uo, un);
+                /* 1 + 4 ETA / lambda: a quotient, nothing to fuse */
+                float qn = fmaf(ROF_ETA, un - img, qq) /
                            (1.0f + 4.0f * ROF_ETA / lambda[k]);
-                qn = qn + ROF_THETA * (qn - qq);
+                qn = fmaf(ROF_THETA, qn - qq, qn);
                 u_out[o * 4 + k] = un;
                 q_out[o * 4 + k] = qn;
                 p_out[o * 8 + k] = Plo[k];

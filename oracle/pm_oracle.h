@@ -25,9 +25,14 @@
  *              empty; key = (lr ? W-1-x : x) of the scattering pixel.
  *
  * Arithmetic contract shared with the HIP engine: IEEE float32, no
- * contraction (built with -ffp-contract=off) except the fused operations
- * written as fmaf() below; division and sqrt correctly rounded; exp, sin,
- * cos and acospi are this file's own Cody-Waite + Taylor implementations.
+ * contraction by the compiler (built with -ffp-contract=off); the fused
+ * operations are written as fmaf(), by one rule: the contractions an
+ * OpenCL compiler with FP_CONTRACT ON forms in the reference's source
+ * expressions (a product operand of a sum or difference fused into it,
+ * the left one first; a built-in dot as the chain fma(x2, y2, fma(x1, y1,
+ * x0 * y0)); never across statements or helper calls).  Division and sqrt
+ * correctly rounded; exp, sin, cos and acospi are this file's own
+ * Cody-Waite + Taylor implementations.
  */
 #ifndef PM_ORACLE_H
 #define PM_ORACLE_H

@@ -36,8 +36,9 @@ def gray_at(img, x, y):
     if x < 0 or x >= W or y < 0 or y >= H:
         return f32(0.0)
     v = rgba(img, x, y)
-    return f32(f32(f32(v[0] * GW[0]) + f32(v[1] * GW[1])) + f32(v[2] * GW[2])) \
-        + f32(v[3] * f32(0.0))
+    # the built-in dot as a chain of fused multiply-adds
+    return fma(v[3], f32(0.0),
+               fma(v[2], GW[2], fma(v[1], GW[1], f32(v[0] * GW[0]))))
 
 
 def weight(d, cw=12.0):
@@ -88,9 +89,11 @@ def signs(g):
 
 def to_normal_z(s):
     sx, sy = f32(1.0 - s[0]), f32(0.0 - s[1])
-    nz = f32(f32(1.0) / f32(math.sqrt(f32(f32(sx * sx + sy * sy) + 1.0))))
+    nz = f32(f32(1.0) / f32(math.sqrt(f32(fma(sx, sx, f32(sy * sy)) +
+                                          f32(1.0)))))
     nx, ny = f32(-sx * nz), f32(-sy * nz)
-    t = f32(f32(f32(1.0) - f32(nx * nx)) - f32(ny * ny))
+    # (1 - nx nx) - ny ny, each product fused into its difference
+    t = fma(-ny, ny, fma(-nx, nx, f32(1.0)))
     return f32(math.sqrt(t)) if t >= 0 else f32("nan")
 
 
@@ -181,17 +184,18 @@ def lusolve(m, b):
     b = [f32(v) for v in b]
     m[1][0] = f32(m[1][0] / m[0][0])
     m[2][0] = f32(m[2][0] / m[0][0])
-    m[1][1] = f32(m[1][1] - f32(m[1][0] * m[0][1]))
-    m[1][2] = f32(m[1][2] - f32(m[1][0] * m[0][2]))
-    m[2][1] = f32(m[2][1] - f32(m[2][0] * m[0][1]))
-    m[2][2] = f32(m[2][2] - f32(m[2][0] * m[0][2]))
+    # x -= a * b as fma(-a, b, x); dot(u, v) = fma(u1, v1, u0 * v0)
+    m[1][1] = fma(-m[1][0], m[0][1], m[1][1])
+    m[1][2] = fma(-m[1][0], m[0][2], m[1][2])
+    m[2][1] = fma(-m[2][0], m[0][1], m[2][1])
+    m[2][2] = fma(-m[2][0], m[0][2], m[2][2])
     m[2][1] = f32(m[2][1] / m[1][1])
-    m[2][2] = f32(m[2][2] - f32(m[2][1] * m[1][2]))
-    b[1] = f32(b[1] - f32(m[1][0] * b[0]))
-    b[2] = f32(b[2] - f32(f32(m[2][0] * b[0]) + f32(m[2][1] * b[1])))
+    m[2][2] = fma(-m[2][1], m[1][2], m[2][2])
+    b[1] = fma(-m[1][0], b[0], b[1])
+    b[2] = f32(b[2] - fma(m[2][1], b[1], f32(m[2][0] * b[0])))
     z = f32(b[2] / m[2][2])
-    yv = f32(f32(b[1] - f32(m[1][2] * z)) / m[1][1])
-    xv = f32(f32(b[0] - f32(f32(m[0][1] * yv) + f32(m[0][2] * z))) / m[0][0])
+    yv = f32(fma(-m[1][2], z, b[1]) / m[1][1])
+    xv = f32(f32(b[0] - fma(m[0][2], z, f32(m[0][1] * yv))) / m[0][0])
     return [xv, yv, z]
 
 

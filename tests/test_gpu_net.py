@@ -230,9 +230,13 @@ def test_setup_proposal_bitexact(oracle, synth):
                 d = np.float32(o[0]) * np.float32(128.0)
                 n = np.array([o[1], o[2] if lr else -o[2], o[3]], np.float32)
                 if d > 0 and np.isfinite(d) and np.isfinite(n).all():
-                    ln = np.sqrt(np.float32(np.float32(n[0] * n[0] +
-                                                       n[1] * n[1]) +
-                                            n[2] * n[2]))
+                    # normalize(): |n| from the built-in dot, a chain of
+                    # fused multiply-adds (exact product in float64)
+                    l2 = np.float32(np.float64(n[1]) * n[1] +
+                                    np.float64(np.float32(n[0] * n[0])))
+                    l2 = np.float32(np.float64(n[2]) * n[2] +
+                                    np.float64(l2))
+                    ln = np.sqrt(l2)
                     n = (n / ln).astype(np.float32) if ln > 0 else n
                     prop[y, x] = oracle.from_disparity_normal(
                         x, y, d if lr else -d, n)

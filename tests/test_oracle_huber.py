@@ -85,11 +85,20 @@ def test_normal_depth_of_fronto_planes(oracle):
 SIGMA, TAU, ETA, THETA = F(4.0), F(0.05), F(8.0), F(0.5)
 
 
+def fma(a, b, c):
+    """elementwise float32 a * b + c with one rounding (the product of two
+    float32 is exact in float64)"""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64) +
+            np.asarray(c, np.float64)).astype(np.float32)
+
+
 def rof_step_numpy(u, p, q, f, dt, lam):
     """PDDROFDiffTensor.cl:43-125 restated with numpy float32 array
     operations: the dual of every position (-1..W-1, -1..H-1) first, then
     the divergence, primal and q updates.  The stored dual is the
-    tensor-applied one, as the kernel writes it."""
+    tensor-applied one, as the kernel writes it.  A product that is an
+    operand of a sum in the kernel's source expression is fused into it
+    (the left one first), as an OpenCL compiler contracts."""
     H, W = u.shape[:2]
     jj, ii = np.mgrid[-1:H, -1:W]
     inside = ((ii >= 0) & (jj >= 0))[..., None]
@@ -102,25 +111,25 @@ def rof_step_numpy(u, p, q, f, dt, lam):
     phi = np.where(inside, p[cj, ci, 4:], F(0))
     dx, dy = u0 - ux, u0 - uy
     d0, d1, d2, d3 = (D[..., k:k + 1] for k in range(4))
-    DTdx = d0 * dx + d2 * dy
-    DTdy = d1 * dx + d3 * dy
-    plo = plo + SIGMA * DTdx
-    phi = phi + SIGMA * DTdy
-    n = plo * plo + phi * phi
+    DTdx = fma(d0, dx, d2 * dy)
+    DTdy = fma(d1, dx, d3 * dy)
+    plo = fma(SIGMA, DTdx, plo)
+    phi = fma(SIGMA, DTdy, phi)
+    n = fma(plo, plo, phi * phi)
     nxyz = np.sqrt((n[..., 0] + n[..., 1]) + n[..., 2])
     proj = np.stack([np.fmax(F(1), nxyz)] * 3 +
                     [np.fmax(F(1), np.sqrt(n[..., 3]))], -1)
     plo, phi = plo / proj, phi / proj
-    lo = d0 * plo + d2 * phi
-    hi = d1 * plo + d3 * phi
+    lo = fma(d0, plo, d2 * phi)
+    hi = fma(d1, plo, d3 * phi)
     Plo, Phi = lo[1:, 1:], hi[1:, 1:]
     Llo, Uhi = lo[1:, :-1], hi[:-1, 1:]
     div = ((Plo - Llo) + Phi) - Uhi
     lam = np.asarray(lam, np.float32)
-    un = (u + TAU * ((lam * f - div) - q)) / (F(1) + TAU * lam)
-    un = un + THETA * (un - u)
-    qn = (q + ETA * (un - f)) / (F(1) + F(4) * ETA / lam)
-    qn = qn + THETA * (qn - q)
+    un = fma(TAU, fma(lam, f, -div) - q, u) / fma(TAU, lam, F(1))
+    un = fma(THETA, un - u, un)
+    qn = fma(ETA, un - f, q) / (F(1) + F(4) * ETA / lam)
+    qn = fma(THETA, qn - q, qn)
     return un, np.concatenate([Plo, Phi], -1), qn
 
 

@@ -38,6 +38,12 @@ _NUMDRAWS_TEXT = ("( ( ( 2 * {P} + 1 ) * ( 2 * {P} + 1 ) - 1 ) + {T} ) * 3"
                   " + 2 * {S}").format(P=0, T=2, S=4)
 
 
+def _fma(a, b, c):
+    """float32 a * b + c with one rounding (the product is exact in
+    float64)"""
+    return F(np.float64(F(a)) * np.float64(F(b)) + np.float64(F(c)))
+
+
 def prop_draw_start(p, it):
     """Stream index of the first propagate draw: the macro's token text
     evaluated with C's precedence for + and * (which Python shares)."""
@@ -117,9 +123,9 @@ def propagate_py(O, t1, t2, R, dmax, lr, it, old, src_other,
                 c = np.array(cand, F).copy()
                 ds = _sdist(O, self_, x, y, lr, dmax)
                 dc = _sdist(O, c, x, y, lr, dmax)
-                mincost = F(F(self_[3] + theta * ds) - theta * dc)
+                mincost = _fma(-theta, dc, _fma(theta, ds, self_[3]))
                 c[3] = O.cost_at(t1, t2, R, x, y, c, float(mincost), **cost)
-                if F(c[3] + theta * dc) <= F(self_[3] + theta * ds):
+                if _fma(theta, dc, c[3]) <= _fma(theta, ds, self_[3]):
                     return c
                 return self_
 
@@ -130,13 +136,14 @@ def propagate_py(O, t1, t2, R, dmax, lr, it, old, src_other,
             if not lr:
                 cand = O.viewprop(cand)
             self_ = attempt(cand)
-            # four random neighbours, offsets trunc((u - .5) * 2 * 5 + .5),
-            # x draw first, clamped to the image (CLK_ADDRESS_CLAMP_TO_EDGE)
+            # four random neighbours, offsets trunc((u - .5) * 2 * 5 + .5)
+            # with the product by 5 fused into the sum, x draw first,
+            # clamped to the image (CLK_ADDRESS_CLAMP_TO_EDGE)
             for i in range(4):
-                ox = int(F(F(F(F(u[2 * i]) - F(0.5)) * F(2.0)) * F(5.0))
-                         + F(0.5))
-                oy = int(F(F(F(F(u[2 * i + 1]) - F(0.5)) * F(2.0)) * F(5.0))
-                         + F(0.5))
+                ox = int(_fma(F(F(F(u[2 * i]) - F(0.5)) * F(2.0)), F(5.0),
+                              F(0.5)))
+                oy = int(_fma(F(F(F(u[2 * i + 1]) - F(0.5)) * F(2.0)),
+                              F(5.0), F(0.5)))
                 nx = min(max(x + ox, 0), W - 1)
                 ny = min(max(y + oy, 0), H - 1)
                 src = fresh if (color == 1 and (nx + ny) % 2 == 0) else old
