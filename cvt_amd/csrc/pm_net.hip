@@ -33,15 +33,18 @@ __device__ __forceinline__ bf16x8 zero8() {
 
 // ---------------------------------------------------------------------
 // convolution: a work-group is one wavefront computing 64 output pixels x
-// 32 output channels as 4 x 2 tiles of 16 x 16.  Per K step of 32 a lane
+// 16 kNT output channels as 4 x kNT tiles of 16 x 16.  Per K step of 32 a lane
 // holds the A operand of its tile row (pixel) and k-group (8 consecutive
 // k: one tap, 8 channels, one 16-byte load from NHWC) and the B operand
 // of its tile column (output channel: 8 consecutive k of its weight row).
 // The accumulator of a lane holds rows 4 g .. 4 g + 3, column lane % 16.
 // ---------------------------------------------------------------------
 constexpr int kConvPix = 64;
-constexpr int kConvCo = 32;
 
+// kNT column tiles of 16 output channels per wavefront: 2 (32 channels)
+// or 4 (64, when the padded output count is a multiple of 64, which
+// halves the A loads per matrix-core op)
+template <int kNT>
 __global__ void __launch_bounds__(64) k_conv(ConvArgs a) {
   const int lane = threadIdx.x;
   const int row = lane & 15, grp = lane >> 4;
@@ -49,7 +52,7 @@ __global__ void __launch_bounds__(64) k_conv(ConvArgs a) {
   const int kpad = conv_kpad(cin), kreal = 9 * cin;
   const long npix = (long)a.N * a.H * a.W;
   const long p0 = (long)blockIdx.x * kConvPix;
-  const int co0 = blockIdx.y * kConvCo;
+  const int co0 = blockIdx.y * (16 * kNT);
   const int H2 = a.up2 ? (a.H + 1) >> 1 : a.H;
   const int W2 = a.up2 ? (a.W + 1) >> 1 : a.W;
   int pn[4], py[4], px[4];
@@ -64,13 +67,12 @@ __global__ void __launch_bounds__(64) k_conv(ConvArgs a) {
     py[mt] = (int)(t % a.H);
     pn[mt] = (int)(t / a.H);
   }
-  f32x4 acc[4][2];
+  f32x4 acc[4][kNT];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const bf16* w0 = a.w + (size_t)(co0 + row) * kpad;
-  const bf16* w1 = a.w + (size_t)(co0 + 16 + row) * kpad;
+    for (int nt = 0; nt < kNT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16* wrow = a.w + (size_t)(co0 + row) * kpad;
   // this lane's k = k0 + 8 grp as (tap, channel), advanced by 32 per step
   int tap = (8 * grp) / cin, ci = (8 * grp) % cin;
   for (int k0 = 0; k0 < kpad; k0 += 32) {
@@ -99,14 +101,16 @@ __global__ void __launch_bounds__(64) k_conv(ConvArgs a) {
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) A[mt] = zero8();
     }
-    const bf16x8 B0 = load8(w0 + k), B1 = load8(w1 + k);
+    bf16x8 B[kNT];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      acc[mt][0] =
-          __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B0, acc[mt][0], 0, 0, 0);
-      acc[mt][1] =
-          __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B1, acc[mt][1], 0, 0, 0);
-    }
+    for (int nt = 0; nt < kNT; ++nt)
+      B[nt] = load8(wrow + (size_t)nt * 16 * kpad + k);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt)
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            A[mt], B[nt], acc[mt][nt], 0, 0, 0);
     ci += 32;
     while (ci >= cin) {
       ci -= cin;
@@ -116,7 +120,7 @@ __global__ void __launch_bounds__(64) k_conv(ConvArgs a) {
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) {
+    for (int nt = 0; nt < kNT; ++nt) {
       const int co = co0 + nt * 16 + row;
       if (co >= a.cout) continue;
       const float b = a.bias ? a.bias[co] : 0.0f;
@@ -136,9 +140,12 @@ __global__ void __launch_bounds__(64) k_conv(ConvArgs a) {
 
 hipError_t launch_conv(hipStream_t st, const ConvArgs& a) {
   const long npix = (long)a.N * a.H * a.W;
-  const dim3 grid((unsigned)((npix + kConvPix - 1) / kConvPix),
-                  (unsigned)(conv_cpad(a.cout) / kConvCo));
-  NET_LAUNCH(k_conv, grid, dim3(64), 0, st, a);
+  const int cpad = conv_cpad(a.cout);
+  const unsigned gx = (unsigned)((npix + kConvPix - 1) / kConvPix);
+  if (cpad % 64 == 0)
+    NET_LAUNCH(k_conv<4>, dim3(gx, (unsigned)(cpad / 64)), dim3(64), 0, st,
+               a);
+  NET_LAUNCH(k_conv<2>, dim3(gx, (unsigned)(cpad / 32)), dim3(64), 0, st, a);
 }
 
 // ---------------------------------------------------------------------
@@ -434,6 +441,86 @@ k_to_bf16(size_t n, const float* __restrict__ x, bf16* __restrict__ y) {
 hipError_t launch_to_bf16(hipStream_t st, size_t n, const float* x, bf16* y) {
   NET_LAUNCH(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
              n, x, y);
+}
+
+// ---------------------------------------------------------------------
+// matching cost volume: the network's explicit stereo evidence.  The
+// packed pair is box-averaged 4 x 4, and at every quarter-resolution pixel
+// channel d of the volume is the mean over the 3 x 3 window of the
+// summed |r, g, b| differences between the first image at (x + i, y + j)
+// and the second at (x + i - d, y + j), d = 0 .. kCvLevels - 1 quarter
+// pixels (0 .. 188 px at full resolution); coordinates are clamped to the
+// image.  No parameters: the network learns what follows it.
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_down4(int N, int H, int W, const bf16* __restrict__ x,
+        bf16* __restrict__ y) {
+  const int Ho = H / 4, Wo = W / 4;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (size_t)N * Ho * Wo) return;
+  const int xo = (int)(i % Wo);
+  const size_t r = i / Wo;
+  const int yo = (int)(r % Ho), n = (int)(r / Ho);
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int dy = 0; dy < 4; ++dy)
+    for (int dx = 0; dx < 4; ++dx) {
+      const bf16x8 v = load8(
+          x + (((size_t)n * H + 4 * yo + dy) * W + 4 * xo + dx) * 8);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) s[c] += (float)v[c];
+    }
+  bf16x8 o;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) o[c] = (bf16)(s[c] * 0.0625f);
+  *reinterpret_cast<bf16x8*>(y + i * 8) = o;
+}
+
+hipError_t launch_down4(hipStream_t st, int N, int H, int W, const bf16* x,
+                        bf16* y) {
+  const size_t n = (size_t)N * (H / 4) * (W / 4);
+  NET_LAUNCH(k_down4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, N,
+             H, W, x, y);
+}
+
+// one thread per (pixel, 8 consecutive disparities)
+__global__ void __launch_bounds__(256)
+k_cost_volume(int N, int H4, int W4, const bf16* __restrict__ d4,
+              bf16* __restrict__ cv) {
+  constexpr int kGroups = kCvLevels / 8;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (size_t)N * H4 * W4 * kGroups) return;
+  const int grp = (int)(i % kGroups);
+  const size_t p = i / kGroups;
+  const int x = (int)(p % W4);
+  const size_t r = p / W4;
+  const int y = (int)(r % H4), n = (int)(r / H4);
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int j = -1; j <= 1; ++j) {
+    const int yy = min(max(y + j, 0), H4 - 1);
+    const bf16* row = d4 + ((size_t)n * H4 + yy) * W4 * 8;
+    for (int t = -1; t <= 1; ++t) {
+      const bf16x8 a = load8(row + (size_t)min(max(x + t, 0), W4 - 1) * 8);
+      const float a0 = (float)a[0], a1 = (float)a[1], a2 = (float)a[2];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int xb = min(max(x + t - (grp * 8 + k), 0), W4 - 1);
+        const bf16x8 b = load8(row + (size_t)xb * 8);
+        acc[k] += (fabsf(a0 - (float)b[4]) + fabsf(a1 - (float)b[5])) +
+                  fabsf(a2 - (float)b[6]);
+      }
+    }
+  }
+  bf16x8 o;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o[k] = (bf16)(acc[k] * (1.0f / 9.0f));
+  *reinterpret_cast<bf16x8*>(cv + p * kCvLevels + grp * 8) = o;
+}
+
+hipError_t launch_cost_volume(hipStream_t st, int N, int H4, int W4,
+                              const bf16* d4, bf16* cv) {
+  const size_t n = (size_t)N * H4 * W4 * (kCvLevels / 8);
+  NET_LAUNCH(k_cost_volume, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+             st, N, H4, W4, d4, cv);
 }
 
 __global__ void __launch_bounds__(256)

@@ -29,10 +29,10 @@ CH = {"x0": (0, 8), "e0a": (0, 32), "e0": (0, 32), "p1": (1, 32),
       "e2": (2, 96), "p3": (3, 96), "b0": (3, 128), "b1": (3, 128),
       "b2": (3, 128), "b3": (3, 128), "d2a": (2, 96), "d2": (2, 96),
       "d1a": (1, 64), "d1": (1, 64), "d0a": (0, 32), "d0": (0, 32),
-      "out": (0, 8)}
+      "out": (0, 8), "cv": (2, 48)}
 CONVS = [("x0", None, "e0a", 1, 1), ("e0a", None, "e0", 1, 1),
          ("p1", None, "e1a", 1, 1), ("e1a", None, "e1", 1, 1),
-         ("p2", None, "e2a", 1, 1), ("e2a", None, "e2", 1, 1),
+         ("p2", "cv", "e2a", 1, 1), ("e2a", None, "e2", 1, 1),
          ("p3", None, "b0", 1, 1), ("b0", None, "b1", 2, 1),
          ("b1", None, "b2", 4, 1), ("b2", None, "b3", 8, 1),
          ("e2", "b3", "d2a", 1, 1), ("d2a", None, "d2", 1, 1),
@@ -75,18 +75,51 @@ def bf(t):
     return t.to(torch.bfloat16).to(torch.float32)
 
 
+CV_LEVELS = 48
+
+
+def ref_cost_volume(x0):
+    """the matching cost volume from the bf16 input (N, 8, H, W): 4 x 4
+    box average (rounded to bf16), then for d = 0..47 the mean over the
+    3 x 3 window of sum_rgb |A(x + i, y + j) - B(x + i - d, y + j)|,
+    coordinates clamped to the image, rounded to bf16"""
+    import torch
+    import torch.nn.functional as F
+    d4 = bf(F.avg_pool2d(x0, 4))
+    N, _, H4, W4 = d4.shape
+    a, b = d4[:, 0:3], d4[:, 4:7]
+    cols = torch.arange(W4)
+    rows = torch.arange(H4)
+    out = []
+    for d in range(CV_LEVELS):
+        acc = torch.zeros((N, H4, W4))
+        for j in (-1, 0, 1):
+            yy = (rows + j).clamp(0, H4 - 1)
+            for t in (-1, 0, 1):
+                xa = (cols + t).clamp(0, W4 - 1)
+                xb = (cols + t - d).clamp(0, W4 - 1)
+                da = a[:, :, yy][:, :, :, xa]
+                db = b[:, :, yy][:, :, :, xb]
+                acc = acc + (da - db).abs().sum(1)
+        out.append(acc / 9.0)
+    return bf(torch.stack(out, 1))
+
+
 def ref_forward(x, ws):
     """x: (N, 8, H, W) float32 CPU tensor; returns the head (N, 8, H, W)"""
     import torch
     import torch.nn.functional as F
     act = {"x0": bf(x)}
+    act["cv"] = ref_cost_volume(act["x0"])
     for op in ORDER:
         if isinstance(op, tuple):
             act[op[1]] = bf(F.avg_pool2d(act[op[0]], 2))
             continue
         in1, in2, o, dil, relu = CONVS[op]
         inp = act[in1]
-        if in2:
+        if in2 == "cv":                   # same resolution, no gradient
+            inp = torch.cat([inp, act[in2]], 1)
+        elif in2:
             up = F.interpolate(act[in2], scale_factor=2, mode="nearest")
             inp = torch.cat([inp, up[..., :inp.shape[2], :inp.shape[3]]], 1)
         w, b = ws[op]
