@@ -278,13 +278,15 @@ BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA peak
 def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
                     steps, warmup):
     """Learned-init PatchMatch on the 720p pair: the proposal network seeds
-    both views (pm_setup_proposal: the network on the pair and on the
+    both views (pm_setup_proposal: one network pass over the pair and the
     mirrored pair, each pixel keeping the proposed plane unless the random
-    init plane costs less), then k refine iterations.  bad-1.0px after
-    k = 0..8 refine iterations; k is the fewest that match the plain
-    8-iteration bad-1.0px (or 8 when none does).  Timed like the plain
-    step: set images from HBM, setup, iterations, LR check, disparity.
-    None when no trained weights file exists."""
+    init plane costs less), then k refine iterations, in the reference's
+    Jacobi sweep and in the red-black schedule.  For each schedule:
+    bad-1.0px after k = 0..8 refine iterations, and k = the fewest that
+    match the plain 8-iteration bad-1.0px (8 when none does), timed like
+    the plain step (set images from HBM, setup, iterations, LR check,
+    disparity).  The line reports the faster matched schedule.  None when
+    no trained weights file exists."""
     import numpy as np
     import torch
 
@@ -294,75 +296,87 @@ def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
         return None
     dD = torch.empty((H, W), dtype=torch.float32, device=dL.device)
     net = ProposalNet(stream=stream.cuda_stream, weights=WEIGHTS)
-    ctx = cvt_amd.Context(radius=RADIUS, disparity_max=DMAX,
-                          stream=stream.cuda_stream, cost_kind=cost_kind,
-                          cost_weight=cost_weight)
-    bad = []
-    with torch.cuda.stream(stream):
-        for k in range(ITERS + 1):
-            ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
-            ctx.setup_proposal(net)
-            if k:
+    runs = {}
+    for name, sched in (("jacobi", cvt_amd.PM_SCHEDULE_JACOBI),
+                        ("red_black", cvt_amd.PM_SCHEDULE_RED_BLACK)):
+        ctx = cvt_amd.Context(radius=RADIUS, disparity_max=DMAX,
+                              stream=stream.cuda_stream, cost_kind=cost_kind,
+                              cost_weight=cost_weight, schedule=sched)
+        bad = []
+        with torch.cuda.stream(stream):
+            for k in range(ITERS + 1):
+                ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
+                ctx.setup_proposal(net)
+                if k:
+                    ctx.iterate(k)
+                ctx.disparity_device(dD.data_ptr(),
+                                     cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP,
+                                     MAXDEG)
+                torch.cuda.synchronize()
+                bad.append(cvt_amd.disparity.bad_pixel_rate(
+                    dD.cpu().numpy(), gt, 1.0))
+            matched = matched_iterations(bad, plain_bad)
+            k = matched if matched is not None else ITERS
+
+            def step(ev=None):
+                ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
+                if ev:
+                    ev[0].record(stream)
+                ctx.setup_proposal(net)
+                if ev:
+                    ev[1].record(stream)
                 ctx.iterate(k)
-            ctx.disparity_device(dD.data_ptr(), cvt_amd.DISPARITY_PP_LRCHECK,
-                                 MAXDISP, MAXDEG)
+                if ev:
+                    ev[2].record(stream)
+                ctx.disparity_device(dD.data_ptr(),
+                                     cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP,
+                                     MAXDEG)
+
+            for _ in range(warmup):
+                step()
+            ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
+                  for _ in range(steps)]
             torch.cuda.synchronize()
-            bad.append(cvt_amd.disparity.bad_pixel_rate(dD.cpu().numpy(), gt,
-                                                        1.0))
-        matched = matched_iterations(bad, plain_bad)
-        k = matched if matched is not None else ITERS
-
-        def step(ev=None):
-            ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
-            if ev:
-                ev[0].record(stream)
-            ctx.setup_proposal(net)
-            if ev:
-                ev[1].record(stream)
-            ctx.iterate(k)
-            if ev:
-                ev[2].record(stream)
-            ctx.disparity_device(dD.data_ptr(), cvt_amd.DISPARITY_PP_LRCHECK,
-                                 MAXDISP, MAXDEG)
-
-        for _ in range(warmup):
-            step()
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
-              for _ in range(steps)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(steps):
-            step(ev[i])
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        disp = dD.cpu().numpy()
-        setup_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
-        iter_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
-        # the network alone: one forward pass over the padded pair (the
-        # setup runs two, the pair and its mirror), best of five
-        Hp, Wp = padded(H, W)
-        x = torch.rand((1, Hp, Wp, 8), device=dL.device)
-        out = torch.empty((1, Hp, Wp, 4), device=dL.device)
-        fwd = []
+            t0 = time.perf_counter()
+            for i in range(steps):
+                step(ev[i])
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            disp = dD.cpu().numpy()
+        ctx.close()
+        runs[name] = {
+            "value": steps / (t1 - t0), "ms_per_step": (t1 - t0) / steps * 1e3,
+            "refine_iterations": k, "matched": matched is not None,
+            "bad_1px": cvt_amd.disparity.bad_pixel_rate(disp, gt, 1.0),
+            "bad_1px_by_refine_iterations": bad,
+            "setup_ms": float(np.mean([e[0].elapsed_time(e[1]) for e in ev])),
+            "iterations_ms": float(np.mean([e[1].elapsed_time(e[2])
+                                            for e in ev]))}
+    # the network alone: one pass over the padded pair and its mirror as a
+    # batch of two, as the setup runs it; best of five after a warm-up
+    Hp, Wp = padded(H, W)
+    x = torch.rand((2, Hp, Wp, 8), device=dL.device)
+    out = torch.empty((2, Hp, Wp, 4), device=dL.device)
+    fwd = []
+    with torch.cuda.stream(stream):
         for _ in range(6):
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            net.forward(x.data_ptr(), 1, Hp, Wp, out.data_ptr())
+            net.forward(x.data_ptr(), 2, Hp, Wp, out.data_ptr())
             e1.record(stream)
             torch.cuda.synchronize()
             fwd.append(e0.elapsed_time(e1))
-        fwd_ms = float(np.min(fwd[1:]))
-    ctx.close()
+    fwd_ms = float(np.min(fwd[1:]))
     net.close()
-    flops = forward_flops(H, W)
+    flops = forward_flops(H, W, N=2)
     tflops = flops / (fwd_ms * 1e-3) / 1e12
-    return {
-        "value": steps / (t1 - t0), "ms_per_step": (t1 - t0) / steps * 1e3,
-        "refine_iterations": k, "matched": matched is not None,
-        "bad_1px": cvt_amd.disparity.bad_pixel_rate(disp, gt, 1.0),
-        "bad_1px_by_refine_iterations": bad,
-        "setup_ms": setup_ms, "iterations_ms": iter_ms,
+    matched = [n for n in runs if runs[n]["matched"]]
+    best = (max(matched, key=lambda n: runs[n]["value"]) if matched
+            else "jacobi")
+    out = dict(runs[best])
+    out.update({
+        "schedule": best, "schedules": runs,
         "weights": os.path.relpath(WEIGHTS, ROOT),
         "roofline_net": {"bound": "mfma", "kernel": "k_conv (forward pass)",
                          "achieved": tflops, "peak": BF16_PEAK_TFLOPS,
@@ -370,7 +384,9 @@ def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
                          "frac": tflops / BF16_PEAK_TFLOPS,
                          "traffic": None,
                          "forward_ms": fwd_ms, "flop_per_forward": flops,
-                         "forwards_per_pair": 2}}
+                         "forward": "one pass over the pair and its mirror "
+                                    "(batch of two)"}})
+    return out
 
 
 # the batched configuration: 64 pairs of seeded uneven sizes
@@ -764,14 +780,17 @@ def main():
             result["ms_per_step"] = learned["ms_per_step"]
             result["config"]["variant"] = "learned"
             result["config"]["refine_iterations"] = k
+            result["config"]["refine_schedule"] = learned["schedule"]
             result["config"]["workload"] = (
                 "learned-init PatchMatch: bf16 proposal network (4-level "
                 "32-64-96-128 U-Net on the pair and its mirror) seeding both "
                 "views with per-pixel fallback to random init, then %d "
-                "refine iteration%s; 1280x720 RGBA f32 pair, 24 planar "
+                "refine iteration%s (%s schedule); 1280x720 RGBA f32 pair, "
+                "24 planar "
                 "cells, disparity 0-128 px, 9x9 patch, %s, LR check 1 px; "
                 "against plain PatchMatch with 8 iterations"
                 % (k, "" if k == 1 else "s",
+                   learned["schedule"].replace("_", "-"),
                    "gamma-10 truncated colour+gradient cost (weight 25.5)"
                    if args.cost == "color_grad" else "census cost (weight 12)"))
             result["quality"] = {"bad_1px": learned["bad_1px"],
