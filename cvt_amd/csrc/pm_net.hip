@@ -138,9 +138,184 @@ __global__ void __launch_bounds__(64) k_conv(ConvArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------
+// LDS-tiled convolution (dilation 1, c1 and c2 multiples of 32): a
+// work-group of four wavefronts computes a 64 x 4 pixel tile of one image
+// (wavefront w: tile row w) x 16 kNT output channels.  The input channels
+// are taken 32 at a time: the chunk's (4 + 2) x (64 + 2) pixel window and
+// its weights for the 9 taps are staged in LDS, and every tap is one K
+// step of 32 read from there, so each input pixel is fetched from memory
+// 1.5 times per chunk instead of 9.  The next chunk's global loads are in
+// flight during the current chunk's matrix-core work.  LDS rows of 64
+// bytes (32 channels) are read 16 bytes per lane by 16 consecutive
+// pixels; the 16-byte slot of channel group g of window column x is
+// g ^ ((x >> 1) & 3), so any 8 consecutive columns hit distinct banks.
+// K is summed chunk by chunk, tap by tap within a chunk.
+// ---------------------------------------------------------------------
+constexpr int kTileX = 64, kTileY = 4;
+constexpr int kWinX = kTileX + 2, kWinY = kTileY + 2;
+constexpr int kWinPieces = kWinY * kWinX * 4;   // 16-byte pieces
+
+__device__ __forceinline__ int lds_slot(int g, int col) {
+  return g ^ ((col >> 1) & 3);
+}
+
+template <int kNT>
+__global__ void __launch_bounds__(256)
+k_conv_lds(ConvArgs a, int tiles_x, int tiles_y) {
+  constexpr int kCO = 16 * kNT;
+  constexpr int kWPieces = 9 * kCO * 4;
+  constexpr int kAIters = (kWinPieces + 255) / 256;
+  constexpr int kBIters = (kWPieces + 255) / 256;
+  __shared__ uint4 sA[kWinPieces];
+  __shared__ uint4 sB[kWPieces];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int row = lane & 15, grp = lane >> 4;
+  int t = blockIdx.x;
+  const int tx = t % tiles_x;
+  t /= tiles_x;
+  const int ty = t % tiles_y, n = t / tiles_y;
+  const int x0 = tx * kTileX, y0 = ty * kTileY;
+  const int co0 = blockIdx.y * kCO;
+  const int cin = a.c1 + a.c2;
+  const int kpad = conv_kpad(cin);
+  const int H2 = a.up2 ? (a.H + 1) >> 1 : a.H;
+  const int W2 = a.up2 ? (a.W + 1) >> 1 : a.W;
+  const int nchunks = cin / 32;
+
+  uint4 ra[kAIters], rb[kBIters];
+  // global loads of chunk c (channels 32 c .. 32 c + 31) into ra, rb
+  auto fetch = [&](int c) {
+    const int cc = 32 * c;
+#pragma unroll
+    for (int i = 0; i < kAIters; ++i) {
+      const int q = tid + 256 * i;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (q < kWinPieces) {
+        const int r = q / (kWinX * 4), rem = q - r * (kWinX * 4);
+        const int xt = rem >> 2, g = rem & 3;
+        const int yy = y0 - 1 + r, xx = x0 - 1 + xt;
+        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+          if (cc < a.c1) {
+            v = *reinterpret_cast<const uint4*>(
+                a.x1 + (((size_t)n * a.H + yy) * a.W + xx) * a.c1 + cc +
+                8 * g);
+          } else {
+            const int y2 = a.up2 ? yy >> 1 : yy, x2 = a.up2 ? xx >> 1 : xx;
+            v = *reinterpret_cast<const uint4*>(
+                a.x2 + (((size_t)n * H2 + y2) * W2 + x2) * a.c2 +
+                (cc - a.c1) + 8 * g);
+          }
+        }
+      }
+      ra[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < kBIters; ++i) {
+      const int q = tid + 256 * i;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (q < kWPieces) {
+        const int tap = q / (kCO * 4), rem = q - tap * (kCO * 4);
+        const int co = rem >> 2, g = rem & 3;
+        v = *reinterpret_cast<const uint4*>(
+            a.w + (size_t)(co0 + co) * kpad + tap * cin + cc + 8 * g);
+      }
+      rb[i] = v;
+    }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int i = 0; i < kAIters; ++i) {
+      const int q = tid + 256 * i;
+      if (q < kWinPieces) {
+        const int r = q / (kWinX * 4), rem = q - r * (kWinX * 4);
+        const int xt = rem >> 2, g = rem & 3;
+        sA[(r * kWinX + xt) * 4 + lds_slot(g, xt)] = ra[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kBIters; ++i) {
+      const int q = tid + 256 * i;
+      if (q < kWPieces) {
+        const int tap = q / (kCO * 4), rem = q - tap * (kCO * 4);
+        const int co = rem >> 2, g = rem & 3;
+        sB[(tap * kCO + co) * 4 + lds_slot(g, co)] = rb[i];
+      }
+    }
+  };
+
+  f32x4 acc[4][kNT];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  fetch(0);
+  for (int c = 0; c < nchunks; ++c) {
+    __syncthreads();  // every wavefront is done reading the previous chunk
+    stage();
+    __syncthreads();
+    if (c + 1 < nchunks) fetch(c + 1);
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap % 3;  // window offsets 0..2
+      bf16x8 A[4], B[kNT];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const int xt = mt * 16 + row + dx;
+        A[mt] = __builtin_bit_cast(
+            bf16x8, sA[((wave + dy) * kWinX + xt) * 4 + lds_slot(grp, xt)]);
+      }
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt) {
+        const int co = nt * 16 + row;
+        B[nt] = __builtin_bit_cast(
+            bf16x8, sB[(tap * kCO + co) * 4 + lds_slot(grp, co)]);
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              A[mt], B[nt], acc[mt][nt], 0, 0, 0);
+    }
+  }
+  const int y = y0 + wave;
+  if (y >= a.H) return;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt) {
+      const int co = co0 + nt * 16 + row;
+      if (co >= a.cout) continue;
+      const float b = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = x0 + mt * 16 + 4 * grp + j;
+        if (x >= a.W) continue;
+        const size_t p = ((size_t)n * a.H + y) * a.W + x;
+        float v = acc[mt][nt][j] + b;
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.yf)
+          a.yf[p * a.ys + co] = v;
+        else
+          a.y[p * a.ys + co] = (bf16)v;
+      }
+    }
+}
+
 hipError_t launch_conv(hipStream_t st, const ConvArgs& a) {
   const long npix = (long)a.N * a.H * a.W;
   const int cpad = conv_cpad(a.cout);
+  if (a.dil == 1 && a.c1 % 32 == 0 && a.c2 % 32 == 0 && a.c1 + a.c2 > 0) {
+    const int tiles_x = (a.W + kTileX - 1) / kTileX;
+    const int tiles_y = (a.H + kTileY - 1) / kTileY;
+    const unsigned g = (unsigned)((long)tiles_x * tiles_y * a.N);
+    if (cpad % 64 == 0)
+      NET_LAUNCH(k_conv_lds<4>, dim3(g, (unsigned)(cpad / 64)), dim3(256), 0,
+                 st, a, tiles_x, tiles_y);
+    NET_LAUNCH(k_conv_lds<2>, dim3(g, (unsigned)(cpad / 32)), dim3(256), 0,
+               st, a, tiles_x, tiles_y);
+  }
   const unsigned gx = (unsigned)((npix + kConvPix - 1) / kConvPix);
   if (cpad % 64 == 0)
     NET_LAUNCH(k_conv<4>, dim3(gx, (unsigned)(cpad / 64)), dim3(64), 0, st,
@@ -444,13 +619,15 @@ hipError_t launch_to_bf16(hipStream_t st, size_t n, const float* x, bf16* y) {
 }
 
 // ---------------------------------------------------------------------
-// matching cost volume: the network's explicit stereo evidence.  The
-// packed pair is box-averaged 4 x 4, and at every quarter-resolution pixel
-// channel d of the volume is the mean over the 3 x 3 window of the
-// summed |r, g, b| differences between the first image at (x + i, y + j)
-// and the second at (x + i - d, y + j), d = 0 .. kCvLevels - 1 quarter
-// pixels (0 .. 188 px at full resolution); coordinates are clamped to the
-// image.  No parameters: the network learns what follows it.
+// matching cost volumes: the network's explicit stereo evidence.  The
+// packed pair is box-averaged (4 x 4 for the coarse volume, 2 x 2 for the
+// fine one), and at every pixel of that resolution channel d of the
+// volume is the mean over the 3 x 3 window of the summed |r, g, b|
+// differences between the first image at (x + i, y + j) and the second at
+// (x + i - d, y + j), d = 0 .. levels - 1 (coarse: 48 quarter pixels,
+// 0 .. 188 px at full resolution; fine: 64 half pixels, 0 .. 126 px);
+// coordinates are clamped to the image.  No parameters: the network
+// learns what follows it.
 // ---------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_down4(int N, int H, int W, const bf16* __restrict__ x,
@@ -482,11 +659,14 @@ hipError_t launch_down4(hipStream_t st, int N, int H, int W, const bf16* x,
              H, W, x, y);
 }
 
-// one thread per (pixel, 8 consecutive disparities)
+// one thread per (pixel, 8 consecutive disparities); levels: a multiple
+// of 8.  Disparities 8 g .. 8 g + 7 over the taps x - 1 .. x + 1 read the
+// second image at x + 1 - 8 g - 9 .. x + 1 - 8 g: ten pixels per window
+// row, loaded once.
 __global__ void __launch_bounds__(256)
-k_cost_volume(int N, int H4, int W4, const bf16* __restrict__ d4,
+k_cost_volume(int N, int H4, int W4, int levels, const bf16* __restrict__ d4,
               bf16* __restrict__ cv) {
-  constexpr int kGroups = kCvLevels / 8;
+  const int kGroups = levels / 8;
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= (size_t)N * H4 * W4 * kGroups) return;
   const int grp = (int)(i % kGroups);
@@ -495,32 +675,48 @@ k_cost_volume(int N, int H4, int W4, const bf16* __restrict__ d4,
   const size_t r = p / W4;
   const int y = (int)(r % H4), n = (int)(r / H4);
   float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const int xb0 = x + 1 - 8 * grp;  // second-image column of t = 1, d = 8 g
   for (int j = -1; j <= 1; ++j) {
     const int yy = min(max(y + j, 0), H4 - 1);
     const bf16* row = d4 + ((size_t)n * H4 + yy) * W4 * 8;
-    for (int t = -1; t <= 1; ++t) {
-      const bf16x8 a = load8(row + (size_t)min(max(x + t, 0), W4 - 1) * 8);
-      const float a0 = (float)a[0], a1 = (float)a[1], a2 = (float)a[2];
+    float ar[3][3], br[10][3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const bf16x8 v = load8(row + (size_t)min(max(x + t - 1, 0), W4 - 1) * 8);
+      ar[t][0] = (float)v[0];
+      ar[t][1] = (float)v[1];
+      ar[t][2] = (float)v[2];
+    }
+    // br[m] = second image at column xb0 - m (clamped)
+#pragma unroll
+    for (int m = 0; m < 10; ++m) {
+      const bf16x8 v = load8(row + (size_t)min(max(xb0 - m, 0), W4 - 1) * 8);
+      br[m][0] = (float)v[4];
+      br[m][1] = (float)v[5];
+      br[m][2] = (float)v[6];
+    }
+    // tap t (x + t - 1) at disparity 8 g + k reads column xb0 - (2 - t) - k
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const int xb = min(max(x + t - (grp * 8 + k), 0), W4 - 1);
-        const bf16x8 b = load8(row + (size_t)xb * 8);
-        acc[k] += (fabsf(a0 - (float)b[4]) + fabsf(a1 - (float)b[5])) +
-                  fabsf(a2 - (float)b[6]);
+        const float* bb = br[2 - t + k];
+        acc[k] += (fabsf(ar[t][0] - bb[0]) + fabsf(ar[t][1] - bb[1])) +
+                  fabsf(ar[t][2] - bb[2]);
       }
-    }
   }
   bf16x8 o;
 #pragma unroll
   for (int k = 0; k < 8; ++k) o[k] = (bf16)(acc[k] * (1.0f / 9.0f));
-  *reinterpret_cast<bf16x8*>(cv + p * kCvLevels + grp * 8) = o;
+  *reinterpret_cast<bf16x8*>(cv + p * levels + grp * 8) = o;
 }
 
 hipError_t launch_cost_volume(hipStream_t st, int N, int H4, int W4,
-                              const bf16* d4, bf16* cv) {
-  const size_t n = (size_t)N * H4 * W4 * (kCvLevels / 8);
+                              int levels, const bf16* d4, bf16* cv) {
+  if (levels < 8 || levels % 8) return hipErrorInvalidValue;
+  const size_t n = (size_t)N * H4 * W4 * (levels / 8);
   NET_LAUNCH(k_cost_volume, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-             st, N, H4, W4, d4, cv);
+             st, N, H4, W4, levels, d4, cv);
 }
 
 __global__ void __launch_bounds__(256)

@@ -29,18 +29,19 @@ using pm::net::conv_kpad;
 // ---------------------------------------------------------------------
 enum Tensor {
   X0, E0A, E0, P1, E1A, E1, P2, E2A, E2, P3, B0, B1, B2, B3,
-  D2A, D2, D1A, D1, D0A, D0, OUT, X4, CV, kTensors
+  D2A, D2, D1A, D1, D0A, D0, OUT, X4, CV, X2, CV1, kTensors
 };
 struct TensorDesc {
   int level, ch;
 };
-// X4: the input box-averaged to quarter resolution; CV: the matching cost
-// volume computed from it (no parameters, no gradient)
+// X4, X2: the input box-averaged to quarter and half resolution; CV,
+// CV1: the coarse and fine matching cost volumes computed from them (no
+// parameters, no gradient)
 constexpr TensorDesc kT[kTensors] = {
     {0, 8},   {0, 32},  {0, 32},  {1, 32},  {1, 64},  {1, 64},  {2, 64},
     {2, 96},  {2, 96},  {3, 96},  {3, 128}, {3, 128}, {3, 128}, {3, 128},
     {2, 96},  {2, 96},  {1, 64},  {1, 64},  {0, 32},  {0, 32},  {0, 8},
-    {2, 8},   {2, pm::net::kCvLevels}};
+    {2, 8},   {2, pm::net::kCvLevels}, {1, 8},  {1, pm::net::kCvFineLevels}};
 
 // conv: in1 (+ in2, upsampled when up2) -> out; OUT is the fp32 head
 struct ConvDesc {
@@ -48,7 +49,7 @@ struct ConvDesc {
 };
 constexpr ConvDesc kC[] = {
     {X0, -1, 0, E0A, 1, 1},  {E0A, -1, 0, E0, 1, 1},
-    {P1, -1, 0, E1A, 1, 1},  {E1A, -1, 0, E1, 1, 1},
+    {P1, CV1, 0, E1A, 1, 1}, {E1A, -1, 0, E1, 1, 1},
     {P2, CV, 0, E2A, 1, 1},  {E2A, -1, 0, E2, 1, 1},
     {P3, -1, 0, B0, 1, 1},   {B0, -1, 0, B1, 2, 1},
     {B1, -1, 0, B2, 4, 1},   {B2, -1, 0, B3, 8, 1},
@@ -64,6 +65,9 @@ constexpr PoolDesc kP[] = {{E0, P1}, {E1, P2}, {E2, P3}};
 // forward order: conv k (k >= 0) or pool -(k + 1)
 constexpr int kOrder[] = {0, 1, -1, 2, 3, -2, 4, 5, -3, 6, 7, 8, 9,
                           10, 11, 12, 13, 14, 15, 16};
+
+// tensors computed from the input alone: no gradient is formed for them
+bool input_only(int t) { return t == X4 || t == CV || t == X2 || t == CV1; }
 
 int conv_cin(int l) {
   return kT[kC[l].in1].ch + (kC[l].in2 >= 0 ? kT[kC[l].in2].ch : 0);
@@ -153,7 +157,7 @@ int alloc_batch(pm_net* n, int N, int H, int W) {
       const long lp = level_pix(N, H, W, kT[t].level);
       const size_t el = (size_t)lp * kT[t].ch;
       if ((rc = ensure(n, n->act[t], el * (t == OUT ? 4 : 2)))) return rc;
-      if (t != X0 && t != X4 && t != CV &&
+      if (t != X0 && !input_only(t) &&
           (rc = ensure(n, n->gact[t], el * 4)))
         return rc;
     }
@@ -205,8 +209,14 @@ int forward(pm_net* n, hipStream_t st) {
   NET_HIP(n, pm::net::launch_down4(st, n->N, n->H, n->W, as<bf16>(n->act[X0]),
                                    as<bf16>(n->act[X4])));
   NET_HIP(n, pm::net::launch_cost_volume(st, n->N, n->H >> 2, n->W >> 2,
-                                         as<bf16>(n->act[X4]),
+                                         kT[CV].ch, as<bf16>(n->act[X4]),
                                          as<bf16>(n->act[CV])));
+  NET_HIP(n, pm::net::launch_avgpool(st, n->N, n->H, n->W, kT[X0].ch,
+                                     as<bf16>(n->act[X0]),
+                                     as<bf16>(n->act[X2])));
+  NET_HIP(n, pm::net::launch_cost_volume(st, n->N, n->H >> 1, n->W >> 1,
+                                         kT[CV1].ch, as<bf16>(n->act[X2]),
+                                         as<bf16>(n->act[CV1])));
   for (int op : kOrder) {
     if (op >= 0) {
       NET_HIP(n, pm::net::launch_conv(st, conv_args(n, op)));
@@ -269,8 +279,9 @@ int backward(pm_net* n, hipStream_t st) {
     NET_HIP(n, pm::net::launch_route_grad(
                    st, a.N, a.H, a.W, a.c1, a.c2, a.up2, as<float>(n->tmp),
                    as<float>(n->gact[d.in1]),
-                   d.in2 >= 0 && d.in2 != CV ? as<float>(n->gact[d.in2])
-                                             : nullptr));
+                   d.in2 >= 0 && !input_only(d.in2)
+                       ? as<float>(n->gact[d.in2])
+                       : nullptr));
   }
   return PM_OK;
 }
@@ -494,7 +505,7 @@ int pm_net_train_step(pm_net* n, const float* d_x, const float* d_target,
   // gradients start at zero: activations (accumulated by their consumers)
   // and parameters (accumulated over the pixels)
   for (int t = 0; t < kTensors; ++t)
-    if (t != X0 && t != OUT && t != X4 && t != CV)
+    if (t != X0 && t != OUT && !input_only(t))
       NET_HIP(n, hipMemsetAsync(n->gact[t].ptr, 0,
                                 (size_t)level_pix(N, H, W, kT[t].level) *
                                     kT[t].ch * 4,

@@ -101,17 +101,20 @@ hipError_t launch_pack_pair(hipStream_t st, const float* left,
                             const float* right, int W, int H, int channels,
                             int stride, int Hp, int Wp, int mirror, bf16* x);
 
-// disparity levels of the matching cost volume (quarter pixels)
+// disparity levels of the matching cost volumes: the coarse one at
+// quarter resolution (quarter pixels), the fine one at half resolution
 constexpr int kCvLevels = 48;
+constexpr int kCvFineLevels = 64;
 
 // 4 x 4 box average of N x H x W x 8 bf16 (H, W multiples of 4)
 hipError_t launch_down4(hipStream_t st, int N, int H, int W, const bf16* x,
                         bf16* y);
 
-// N x H4 x W4 x kCvLevels bf16 matching costs of a downsampled pair
-// (N x H4 x W4 x 8: first image rgba, second image rgba)
+// N x H4 x W4 x levels bf16 matching costs of a downsampled pair
+// (N x H4 x W4 x 8: first image rgba, second image rgba); levels a
+// multiple of 8
 hipError_t launch_cost_volume(hipStream_t st, int N, int H4, int W4,
-                              const bf16* d4, bf16* cv);
+                              int levels, const bf16* d4, bf16* cv);
 
 }  // namespace net
 

@@ -24,9 +24,10 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)),
 _MAGIC = b"PMNETW1\0"
 
 # the network's convolutions (pm_net_capi.cpp, kC): input channels, output
-# channels and resolution level (2^-level of the padded input); the third
-# level's first convolution also reads the 48-level matching cost volume
-CONVS = [(8, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1),
+# channels and resolution level (2^-level of the padded input); the first
+# convolutions of the second and third levels also read the 64-level fine
+# and the 48-level coarse matching cost volume
+CONVS = [(8, 32, 0), (32, 32, 0), (32 + 64, 64, 1), (64, 64, 1),
          (64 + 48, 96, 2),
          (96, 96, 2), (96, 128, 3), (128, 128, 3), (128, 128, 3),
          (128, 128, 3), (96 + 128, 96, 2), (96, 96, 2), (64 + 96, 64, 1),

@@ -16,10 +16,11 @@
  * bottleneck 4 convolutions with dilations 1, 2, 4, 8), a decoder of
  * nearest upsampling, concatenation with the encoder's level and 2
  * convolutions per level, and a 3x3 head of 4 outputs: disparity / 128
- * and the normal (nx, ny, nz).  The third level's first convolution also
- * reads a fixed matching cost volume: at quarter resolution, 48 levels of
- * the 3x3-window sum of absolute colour differences between the 4 x 4
- * box-averaged images at horizontal shifts of 0..47 quarter pixels.  bf16 weights and activations, fp32
+ * and the normal (nx, ny, nz).  The first convolutions of the second and
+ * third levels also read fixed matching cost volumes: 64 levels at half
+ * resolution and 48 at quarter resolution of the 3x3-window mean of
+ * absolute colour differences between the 2 x 2 (4 x 4) box-averaged
+ * images at horizontal shifts of 0..63 half (0..47 quarter) pixels.  bf16 weights and activations, fp32
  * accumulation on the matrix cores; fp32 master weights and Adam moments
  * for training.  Inputs are padded to multiples of 8 (edge replicated).
  *

@@ -29,9 +29,9 @@ CH = {"x0": (0, 8), "e0a": (0, 32), "e0": (0, 32), "p1": (1, 32),
       "e2": (2, 96), "p3": (3, 96), "b0": (3, 128), "b1": (3, 128),
       "b2": (3, 128), "b3": (3, 128), "d2a": (2, 96), "d2": (2, 96),
       "d1a": (1, 64), "d1": (1, 64), "d0a": (0, 32), "d0": (0, 32),
-      "out": (0, 8), "cv": (2, 48)}
+      "out": (0, 8), "cv": (2, 48), "cv1": (1, 64)}
 CONVS = [("x0", None, "e0a", 1, 1), ("e0a", None, "e0", 1, 1),
-         ("p1", None, "e1a", 1, 1), ("e1a", None, "e1", 1, 1),
+         ("p1", "cv1", "e1a", 1, 1), ("e1a", None, "e1", 1, 1),
          ("p2", "cv", "e2a", 1, 1), ("e2a", None, "e2", 1, 1),
          ("p3", None, "b0", 1, 1), ("b0", None, "b1", 2, 1),
          ("b1", None, "b2", 4, 1), ("b2", None, "b3", 8, 1),
@@ -75,23 +75,20 @@ def bf(t):
     return t.to(torch.bfloat16).to(torch.float32)
 
 
-CV_LEVELS = 48
-
-
-def ref_cost_volume(x0):
-    """the matching cost volume from the bf16 input (N, 8, H, W): 4 x 4
-    box average (rounded to bf16), then for d = 0..47 the mean over the
-    3 x 3 window of sum_rgb |A(x + i, y + j) - B(x + i - d, y + j)|,
-    coordinates clamped to the image, rounded to bf16"""
+def ref_cost_volume(x0, factor, levels):
+    """a matching cost volume from the bf16 input (N, 8, H, W): factor x
+    factor box average (rounded to bf16), then for d = 0..levels-1 the
+    mean over the 3 x 3 window of sum_rgb |A(x + i, y + j) - B(x + i - d,
+    y + j)|, coordinates clamped to the image, rounded to bf16"""
     import torch
     import torch.nn.functional as F
-    d4 = bf(F.avg_pool2d(x0, 4))
+    d4 = bf(F.avg_pool2d(x0, factor))
     N, _, H4, W4 = d4.shape
     a, b = d4[:, 0:3], d4[:, 4:7]
     cols = torch.arange(W4)
     rows = torch.arange(H4)
     out = []
-    for d in range(CV_LEVELS):
+    for d in range(levels):
         acc = torch.zeros((N, H4, W4))
         for j in (-1, 0, 1):
             yy = (rows + j).clamp(0, H4 - 1)
@@ -110,14 +107,15 @@ def ref_forward(x, ws):
     import torch
     import torch.nn.functional as F
     act = {"x0": bf(x)}
-    act["cv"] = ref_cost_volume(act["x0"])
+    act["cv"] = ref_cost_volume(act["x0"], 4, 48)
+    act["cv1"] = ref_cost_volume(act["x0"], 2, 64)
     for op in ORDER:
         if isinstance(op, tuple):
             act[op[1]] = bf(F.avg_pool2d(act[op[0]], 2))
             continue
         in1, in2, o, dil, relu = CONVS[op]
         inp = act[in1]
-        if in2 == "cv":                   # same resolution, no gradient
+        if in2 in ("cv", "cv1"):          # same resolution, no gradient
             inp = torch.cat([inp, act[in2]], 1)
         elif in2:
             up = F.interpolate(act[in2], scale_factor=2, mode="nearest")
@@ -154,10 +152,12 @@ def test_param_layout_matches():
         assert net.num_params == layout()[1]
 
 
-def test_forward_matches_torch_reference():
+# sizes: whole 64 x 4 pixel tiles at the first level; partial tiles in x
+# at every level and partial rows of tiles at the deeper ones
+@pytest.mark.parametrize("N,H,W", [(2, 48, 64), (1, 40, 200)])
+def test_forward_matches_torch_reference(N, H, W):
     import torch
     from cvt_amd import synth_gpu
-    N, H, W = 2, 48, 64
     x, _ = batch(synth_gpu, N, H, W, seed=5)
     out = torch.empty((N, H, W, 4), device="cuda")
     with ProposalNet(seed=7) as net:
