@@ -591,6 +591,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the engine's plain copy kernel over 512 "
                          "MiB (calibration run for the HBM byte counters)")
+    ap.add_argument("--net-probe", action="store_true",
+                    help="only run the proposal network's forward pass over "
+                         "the padded 720p pair and its mirror (batch of "
+                         "two), 12 passes (profiling run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PatchMatch engine runs on a "
@@ -626,6 +630,29 @@ def main():
                           "copy_probe_bytes_written": nbytes,
                           "avg_ms": ms,
                           "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return 0
+
+    if args.net_probe:
+        from cvt_amd.net import WEIGHTS, ProposalNet, forward_flops, padded
+        Hp, Wp = padded(H, W)
+        x = torch.rand((2, Hp, Wp, 8), device="cuda:0")
+        out = torch.empty((2, Hp, Wp, 4), device="cuda:0")
+        stream = torch.cuda.Stream()
+        with ProposalNet(stream=stream.cuda_stream, weights=WEIGHTS) as net:
+            with torch.cuda.stream(stream):
+                net.forward(x.data_ptr(), 2, Hp, Wp, out.data_ptr())
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                reps = 11
+                e0.record(stream)
+                for _ in range(reps):
+                    net.forward(x.data_ptr(), 2, Hp, Wp, out.data_ptr())
+                e1.record(stream)
+                torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        print(json.dumps({"net_forward_ms": ms, "passes": reps,
+                          "TFLOP_per_s": forward_flops(H, W, N=2) /
+                          (ms * 1e-3) / 1e12}))
         return 0
 
     log("bench: generating the 1280x720 pair")
