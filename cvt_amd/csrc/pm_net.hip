@@ -502,23 +502,33 @@ hipError_t launch_route_grad(hipStream_t st, int N, int H, int W, int c1,
 __global__ void __launch_bounds__(256)
 k_avgpool(int N, int H, int W, int C, const bf16* __restrict__ x,
           bf16* __restrict__ y) {
-  const int Ho = H / 2, Wo = W / 2;
+  // a thread: 8 consecutive channels of one output pixel (C % 8 == 0)
+  const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (size_t)N * Ho * Wo * C) return;
-  const int c = (int)(i % C);
-  const size_t q = i / C;
+  if (i >= (size_t)N * Ho * Wo * C8) return;
+  const int c = (int)(i % C8) * 8;
+  const size_t q = i / C8;
   const int xo = (int)(q % Wo);
   const size_t r = q / Wo;
   const int yo = (int)(r % Ho), n = (int)(r / Ho);
   const size_t b = (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
-  const float s = ((float)x[b] + (float)x[b + C]) +
-                  ((float)x[b + (size_t)W * C] + (float)x[b + (size_t)W * C + C]);
-  y[i] = (bf16)(0.25f * s);
+  const bf16x8 v00 = load8(x + b), v01 = load8(x + b + C);
+  const bf16x8 v10 = load8(x + b + (size_t)W * C);
+  const bf16x8 v11 = load8(x + b + (size_t)W * C + C);
+  bf16x8 o;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float s = ((float)v00[k] + (float)v01[k]) +
+                    ((float)v10[k] + (float)v11[k]);
+    o[k] = (bf16)(0.25f * s);
+  }
+  *reinterpret_cast<bf16x8*>(y + q * C + c) = o;
 }
 
 hipError_t launch_avgpool(hipStream_t st, int N, int H, int W, int C,
                           const bf16* x, bf16* y) {
-  const size_t n = (size_t)N * (H / 2) * (W / 2) * C;
+  if (C % 8) return hipErrorInvalidValue;
+  const size_t n = (size_t)N * (H / 2) * (W / 2) * (C / 8);
   NET_LAUNCH(k_avgpool, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
              N, H, W, C, x, y);
 }
@@ -659,64 +669,86 @@ hipError_t launch_down4(hipStream_t st, int N, int H, int W, const bf16* x,
              H, W, x, y);
 }
 
-// one thread per (pixel, 8 consecutive disparities); levels: a multiple
-// of 8.  Disparities 8 g .. 8 g + 7 over the taps x - 1 .. x + 1 read the
-// second image at x + 1 - 8 g - 9 .. x + 1 - 8 g: ten pixels per window
-// row, loaded once.
+// A work-group computes every level of 64 consecutive pixels of one row.
+// It first stages, as float4 (r, g, b, 0) with clamped coordinates, the
+// three window rows of the first image over columns x0 - 1 .. x0 + 64 and
+// of the second image over x0 - levels .. x0 + 64 (the columns any level
+// of the segment reads); then an item is a (pixel, 8 consecutive levels):
+// levels 8 g .. 8 g + 7 over the taps x - 1 .. x + 1 read the second image
+// at x + 1 - 8 g - 9 .. x + 1 - 8 g, ten pixels per window row.  levels: a
+// multiple of 8, at most kCvMaxLevels.
+constexpr int kCvSeg = 64;
+constexpr int kCvMaxLevels = 64;
+constexpr int kCvAW = kCvSeg + 2;                  // first-image columns
+constexpr int kCvBW = kCvSeg + 1 + kCvMaxLevels;   // second-image columns
+
 __global__ void __launch_bounds__(256)
-k_cost_volume(int N, int H4, int W4, int levels, const bf16* __restrict__ d4,
-              bf16* __restrict__ cv) {
-  const int kGroups = levels / 8;
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (size_t)N * H4 * W4 * kGroups) return;
-  const int grp = (int)(i % kGroups);
-  const size_t p = i / kGroups;
-  const int x = (int)(p % W4);
-  const size_t r = p / W4;
-  const int y = (int)(r % H4), n = (int)(r / H4);
-  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  const int xb0 = x + 1 - 8 * grp;  // second-image column of t = 1, d = 8 g
-  for (int j = -1; j <= 1; ++j) {
-    const int yy = min(max(y + j, 0), H4 - 1);
+k_cost_volume(int N, int H4, int W4, int levels, int segs,
+              const bf16* __restrict__ d4, bf16* __restrict__ cv) {
+  __shared__ float4 sa[3][kCvAW];
+  __shared__ float4 sb[3][kCvBW];
+  int t = blockIdx.x;
+  const int seg = t % segs;
+  t /= segs;
+  const int y = t % H4, n = t / H4;
+  const int x0 = seg * kCvSeg;
+  const int bw = kCvSeg + 1 + levels;  // second-image columns x0 - levels ..
+  for (int i = threadIdx.x; i < 3 * (kCvAW + bw); i += 256) {
+    const int j = i / (kCvAW + bw), c = i - j * (kCvAW + bw);
+    const int yy = min(max(y + j - 1, 0), H4 - 1);
     const bf16* row = d4 + ((size_t)n * H4 + yy) * W4 * 8;
-    float ar[3][3], br[10][3];
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      const bf16x8 v = load8(row + (size_t)min(max(x + t - 1, 0), W4 - 1) * 8);
-      ar[t][0] = (float)v[0];
-      ar[t][1] = (float)v[1];
-      ar[t][2] = (float)v[2];
+    if (c < kCvAW) {
+      const bf16x8 v = load8(row + (size_t)min(max(x0 - 1 + c, 0), W4 - 1) * 8);
+      sa[j][c] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.0f);
+    } else {
+      const int cb = c - kCvAW;
+      const bf16x8 v =
+          load8(row + (size_t)min(max(x0 - levels + cb, 0), W4 - 1) * 8);
+      sb[j][cb] = make_float4((float)v[4], (float)v[5], (float)v[6], 0.0f);
     }
-    // br[m] = second image at column xb0 - m (clamped)
-#pragma unroll
-    for (int m = 0; m < 10; ++m) {
-      const bf16x8 v = load8(row + (size_t)min(max(xb0 - m, 0), W4 - 1) * 8);
-      br[m][0] = (float)v[4];
-      br[m][1] = (float)v[5];
-      br[m][2] = (float)v[6];
-    }
-    // tap t (x + t - 1) at disparity 8 g + k reads column xb0 - (2 - t) - k
-#pragma unroll
-    for (int t = 0; t < 3; ++t)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const float* bb = br[2 - t + k];
-        acc[k] += (fabsf(ar[t][0] - bb[0]) + fabsf(ar[t][1] - bb[1])) +
-                  fabsf(ar[t][2] - bb[2]);
-      }
   }
-  bf16x8 o;
+  __syncthreads();
+  const int groups = levels / 8;
+  for (int item = threadIdx.x; item < kCvSeg * groups; item += 256) {
+    const int xl = item % kCvSeg, grp = item / kCvSeg;
+    const int x = x0 + xl;
+    if (x >= W4) continue;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    // second-image column of t = 1, d = 8 g, as an index into sb
+    const int b0 = xl + 1 - 8 * grp + levels;
+    for (int j = 0; j < 3; ++j) {
+      float4 ar[3], br[10];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) o[k] = (bf16)(acc[k] * (1.0f / 9.0f));
-  *reinterpret_cast<bf16x8*>(cv + p * levels + grp * 8) = o;
+      for (int tt = 0; tt < 3; ++tt) ar[tt] = sa[j][xl + tt];
+      // br[m] = second image at column x + 1 - 8 g - m
+#pragma unroll
+      for (int m = 0; m < 10; ++m) br[m] = sb[j][b0 - m];
+      // tap tt (x + tt - 1) at level 8 g + k reads br[2 - tt + k]
+#pragma unroll
+      for (int tt = 0; tt < 3; ++tt)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const float4 bb = br[2 - tt + k];
+          acc[k] += (fabsf(ar[tt].x - bb.x) + fabsf(ar[tt].y - bb.y)) +
+                    fabsf(ar[tt].z - bb.z);
+        }
+    }
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = (bf16)(acc[k] * (1.0f / 9.0f));
+    const size_t p = ((size_t)n * H4 + y) * W4 + x;
+    *reinterpret_cast<bf16x8*>(cv + p * levels + grp * 8) = o;
+  }
 }
 
 hipError_t launch_cost_volume(hipStream_t st, int N, int H4, int W4,
                               int levels, const bf16* d4, bf16* cv) {
-  if (levels < 8 || levels % 8) return hipErrorInvalidValue;
-  const size_t n = (size_t)N * H4 * W4 * (levels / 8);
-  NET_LAUNCH(k_cost_volume, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-             st, N, H4, W4, levels, d4, cv);
+  if (levels < 8 || levels % 8 || levels > kCvMaxLevels)
+    return hipErrorInvalidValue;
+  const int segs = (W4 + kCvSeg - 1) / kCvSeg;
+  const size_t blocks = (size_t)N * H4 * segs;
+  NET_LAUNCH(k_cost_volume, dim3((unsigned)blocks), dim3(256), 0, st, N, H4,
+             W4, levels, segs, d4, cv);
 }
 
 __global__ void __launch_bounds__(256)

@@ -205,7 +205,9 @@ pm::net::ConvArgs conv_args(pm_net* n, int l) {
   return a;
 }
 
-int forward(pm_net* n, hipStream_t st) {
+// out: when set, the head writes its 4 real outputs there (N x H x W x 4
+// fp32) instead of into act[OUT]
+int forward(pm_net* n, hipStream_t st, float* out = nullptr) {
   NET_HIP(n, pm::net::launch_down4(st, n->N, n->H, n->W, as<bf16>(n->act[X0]),
                                    as<bf16>(n->act[X4])));
   NET_HIP(n, pm::net::launch_cost_volume(st, n->N, n->H >> 2, n->W >> 2,
@@ -219,7 +221,13 @@ int forward(pm_net* n, hipStream_t st) {
                                          as<bf16>(n->act[CV1])));
   for (int op : kOrder) {
     if (op >= 0) {
-      NET_HIP(n, pm::net::launch_conv(st, conv_args(n, op)));
+      pm::net::ConvArgs a = conv_args(n, op);
+      if (out && kC[op].out == OUT) {
+        a.yf = out;
+        a.cout = PM_NET_OUT_CHANNELS;
+        a.ys = PM_NET_OUT_CHANNELS;
+      }
+      NET_HIP(n, pm::net::launch_conv(st, a));
     } else {
       const PoolDesc& p = kP[-op - 1];
       const int lev = kT[p.in].level;
@@ -478,12 +486,7 @@ int pm_net_forward(pm_net* n, const float* d_x, int N, int H, int W,
   const size_t npix = (size_t)N * H * W;
   NET_HIP(n, pm::net::launch_to_bf16(n->stream, npix * PM_NET_IN_CHANNELS, d_x,
                                      as<bf16>(n->act[X0])));
-  if ((rc = forward(n, n->stream))) return rc;
-  // the head's first 4 of 8 channels
-  NET_HIP(n, hipMemcpy2DAsync(d_out, PM_NET_OUT_CHANNELS * 4, n->act[OUT].ptr,
-                              kT[OUT].ch * 4, PM_NET_OUT_CHANNELS * 4, npix,
-                              hipMemcpyDeviceToDevice, n->stream));
-  return PM_OK;
+  return forward(n, n->stream, d_out);
 }
 
 int pm_net_train_step(pm_net* n, const float* d_x, const float* d_target,

@@ -71,7 +71,7 @@ hipError_t launch_route_grad(hipStream_t st, int N, int H, int W, int c1,
                              float* g2);
 
 // 2 x 2 average pooling, bf16 (N x H x W x C -> N x H/2 x W/2 x C, H and
-// W even), and its gradient: gin += 0.25 gout
+// W even, C a multiple of 8), and its gradient: gin += 0.25 gout
 hipError_t launch_avgpool(hipStream_t st, int N, int H, int W, int C,
                           const bf16* x, bf16* y);
 hipError_t launch_avgpool_grad(hipStream_t st, int N, int H, int W, int C,
@@ -112,7 +112,7 @@ hipError_t launch_down4(hipStream_t st, int N, int H, int W, const bf16* x,
 
 // N x H4 x W4 x levels bf16 matching costs of a downsampled pair
 // (N x H4 x W4 x 8: first image rgba, second image rgba); levels a
-// multiple of 8
+// multiple of 8, at most 64
 hipError_t launch_cost_volume(hipStream_t st, int N, int H4, int W4,
                               int levels, const bf16* d4, bf16* cv);
 
