@@ -492,15 +492,16 @@ def matched_iterations(bad, target):
 
 
 # the batched serving mix: 16 pairs of seeded uneven sizes
-SERVE_PAIRS, SERVE_SEED, SERVE_REFINE = 16, 5, 2
+SERVE_PAIRS, SERVE_SEED = 16, 5
 
 
 def serving_section(stream, cost_kind, cost_weight, runs=3):
     """Batched learned serving mix: 16 RGBA pairs from the GPU generator,
     width uniform in 320-1920, height in 240-1080, disparity range 0.1 x
-    width (up to 192 px); the proposal network seeds every pair and 2
-    refine iterations follow, all in one graph launch, next to plain
-    8-iteration batched PatchMatch on the same pairs.  None without a
+    width (up to 192 px); plain 8-iteration batched PatchMatch on them,
+    then the proposal network seeding every pair and k refine iterations,
+    k the fewest whose pooled bad-1.0px (from the on-device error counts)
+    is at most plain's; each run one graph launch.  None without a
     trained weights file."""
     import numpy as np
     import torch
@@ -529,32 +530,49 @@ def serving_section(stream, cost_kind, cost_weight, runs=3):
             Batch(pairs, radius=RADIUS, cost_kind=cost_kind,
                   cost_weight=cost_weight, stream=stream.cuda_stream) as b:
         b.autotune(reps=2)
-        for name, seeded, iters in (("plain", False, ITERS),
-                                    ("learned", True, SERVE_REFINE)):
-            b.set_proposal(net if seeded else None)
-            b.run(iters, cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP, MAXDEG, 1.0,
-                  0.0)
+        pp = cvt_amd.DISPARITY_PP_LRCHECK
+
+        def pooled_bad():
+            c = b.error_counts().astype(np.float64)
+            return (float(c[:, 0].sum() / c[:, 2].sum()),
+                    float(np.mean(c[:, 0] / c[:, 2])))
+
+        def timed(iters):
+            b.run(iters, pp, MAXDISP, MAXDEG, 1.0, 0.0)
             b.sync()
             t0 = time.perf_counter()
             for _ in range(runs):
-                b.run(iters, cvt_amd.DISPARITY_PP_LRCHECK, MAXDISP, MAXDEG,
-                      1.0, 0.0)
+                b.run(iters, pp, MAXDISP, MAXDEG, 1.0, 0.0)
             b.sync()
             t1 = time.perf_counter()
-            c = b.error_counts().astype(np.float64)
-            out[name] = {"value": SERVE_PAIRS * runs / (t1 - t0),
-                         "unit": "pairs/s",
-                         "ms_per_batch": (t1 - t0) / runs * 1e3,
-                         "iterations": iters,
-                         "bad_1px_mean": float(np.mean(c[:, 0] / c[:, 2])),
-                         "bad_1px_pooled": float(c[:, 0].sum() /
-                                                 c[:, 2].sum())}
+            pooled, mean = pooled_bad()
+            return {"value": SERVE_PAIRS * runs / (t1 - t0), "unit": "pairs/s",
+                    "ms_per_batch": (t1 - t0) / runs * 1e3,
+                    "iterations": iters, "bad_1px_mean": mean,
+                    "bad_1px_pooled": pooled}
+
+        b.set_proposal(None)
+        out["plain"] = timed(ITERS)
+        # learned: bad-1px after k = 0, 1, ... refine iterations until it
+        # reaches plain's
+        b.set_proposal(net)
+        bad = []
+        for k in range(ITERS + 1):
+            b.run(k, pp, MAXDISP, MAXDEG, 1.0, 0.0)
+            bad.append(pooled_bad()[0])
+            if matched_iterations(bad, out["plain"]["bad_1px_pooled"]):
+                break
+        matched = matched_iterations(bad, out["plain"]["bad_1px_pooled"])
+        out["learned"] = timed(matched if matched is not None else ITERS)
+        out["learned"]["matched"] = matched is not None
+        out["learned"]["bad_1px_pooled_by_refine_iterations"] = bad
         tuning = b.tuning
     return {"workload": "batched serving mix: 16 RGBA f32 pairs, width "
                         "320-1920, height 240-1080 (seeded), 24 cells, "
                         "disparity 0.1 x width (<= 192 px), 9x9 patch, LR "
-                        "check 1 px; learned: proposal network + 2 refine "
-                        "iterations, plain: 8 iterations; one graph launch "
+                        "check 1 px; learned: proposal network + the "
+                        "fewest refine iterations whose pooled bad-1px is at "
+                        "most plain's, plain: 8 iterations; one graph launch "
                         "per batch",
             "learned": out["learned"], "plain": out["plain"],
             "speedup_vs_plain": out["learned"]["value"] /
