@@ -303,11 +303,19 @@ __device__ __forceinline__ void conv_lds_block(const ConvArgs& a, int tiles_x,
     }
 }
 
-// 32 output channels per work-group: built for three wavefronts per SIMD
-// (168 VGPRs, a few spilled), which measured 1.76 -> 1.70 ms for the 720p
-// forward pass against the unconstrained build (two per SIMD); the
+// 32 output channels per work-group.  With one input chunk (32 input
+// channels) nothing overlaps a work-group's fetch, so that form is built
+// for three wavefronts per SIMD (168 VGPRs, a few spilled); with more
+// chunks the spills cost more than the extra wavefront gains and the
+// unconstrained build (two per SIMD) is used.  At 720p: the single-chunk
+// layers 143 -> 115 us, the others 3-12 % slower at three.  The
 // 64-channel form cannot reach three and keeps two.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_conv_lds32_1chunk(ConvArgs a, int tiles_x, int tiles_y) {
+  conv_lds_block<2>(a, tiles_x, tiles_y);
+}
+
+__global__ void __launch_bounds__(256)
 k_conv_lds32(ConvArgs a, int tiles_x, int tiles_y) {
   conv_lds_block<2>(a, tiles_x, tiles_y);
 }
@@ -327,6 +335,9 @@ hipError_t launch_conv(hipStream_t st, const ConvArgs& a) {
     if (cpad % 64 == 0)
       NET_LAUNCH(k_conv_lds64, dim3(g, (unsigned)(cpad / 64)), dim3(256), 0,
                  st, a, tiles_x, tiles_y);
+    if (a.c1 + a.c2 == 32)
+      NET_LAUNCH(k_conv_lds32_1chunk, dim3(g, (unsigned)(cpad / 32)),
+                 dim3(256), 0, st, a, tiles_x, tiles_y);
     NET_LAUNCH(k_conv_lds32, dim3(g, (unsigned)(cpad / 32)), dim3(256), 0,
                st, a, tiles_x, tiles_y);
   }
