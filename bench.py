@@ -280,13 +280,15 @@ def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
     """Learned-init PatchMatch on the 720p pair: the proposal network seeds
     both views (pm_setup_proposal: one network pass over the pair and the
     mirrored pair, each pixel keeping the proposed plane unless the random
-    init plane costs less), then k refine iterations, in the reference's
-    Jacobi sweep and in the red-black schedule.  For each schedule:
-    bad-1.0px after k = 0..8 refine iterations, and k = the fewest that
-    match the plain 8-iteration bad-1.0px (8 when none does), timed like
-    the plain step (set images from HBM, setup, iterations, LR check,
-    disparity).  The line reports the faster matched schedule.  None when
-    no trained weights file exists."""
+    init plane costs less) or the left view only (pm_setup_proposal_views
+    with views = 1: a pass over the pair, the right view from the random
+    init), then k refine iterations, in the reference's Jacobi sweep and in
+    the red-black schedule.  For each of the four: bad-1.0px after k =
+    0..8 refine iterations, and k = the fewest that match the plain
+    8-iteration bad-1.0px (8 when none does), timed like the plain step
+    (set images from HBM, setup, iterations, LR check, disparity).  The
+    line reports the fastest matched one.  None when no trained weights
+    file exists."""
     import numpy as np
     import torch
 
@@ -297,8 +299,11 @@ def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
     dD = torch.empty((H, W), dtype=torch.float32, device=dL.device)
     net = ProposalNet(stream=stream.cuda_stream, weights=WEIGHTS)
     runs = {}
-    for name, sched in (("jacobi", cvt_amd.PM_SCHEDULE_JACOBI),
-                        ("red_black", cvt_amd.PM_SCHEDULE_RED_BLACK)):
+    for name, sched, views in (
+            ("jacobi", cvt_amd.PM_SCHEDULE_JACOBI, 2),
+            ("red_black", cvt_amd.PM_SCHEDULE_RED_BLACK, 2),
+            ("jacobi_left_seeded", cvt_amd.PM_SCHEDULE_JACOBI, 1),
+            ("red_black_left_seeded", cvt_amd.PM_SCHEDULE_RED_BLACK, 1)):
         ctx = cvt_amd.Context(radius=RADIUS, disparity_max=DMAX,
                               stream=stream.cuda_stream, cost_kind=cost_kind,
                               cost_weight=cost_weight, schedule=sched)
@@ -306,7 +311,7 @@ def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
         with torch.cuda.stream(stream):
             for k in range(ITERS + 1):
                 ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
-                ctx.setup_proposal(net)
+                ctx.setup_proposal(net, views)
                 if k:
                     ctx.iterate(k)
                 ctx.disparity_device(dD.data_ptr(),
@@ -322,7 +327,7 @@ def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
                 ctx.set_images_device(dL.data_ptr(), dR.data_ptr(), W, H, 4)
                 if ev:
                     ev[0].record(stream)
-                ctx.setup_proposal(net)
+                ctx.setup_proposal(net, views)
                 if ev:
                     ev[1].record(stream)
                 ctx.iterate(k)
@@ -346,34 +351,37 @@ def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
         ctx.close()
         runs[name] = {
             "value": steps / (t1 - t0), "ms_per_step": (t1 - t0) / steps * 1e3,
+            "seeded_views": views,
             "refine_iterations": k, "matched": matched is not None,
             "bad_1px": cvt_amd.disparity.bad_pixel_rate(disp, gt, 1.0),
             "bad_1px_by_refine_iterations": bad,
             "setup_ms": float(np.mean([e[0].elapsed_time(e[1]) for e in ev])),
             "iterations_ms": float(np.mean([e[1].elapsed_time(e[2])
                                             for e in ev]))}
-    # the network alone: one pass over the padded pair and its mirror as a
-    # batch of two, as the setup runs it; best of five after a warm-up
+    matched = [n for n in runs if runs[n]["matched"]]
+    best = (max(matched, key=lambda n: runs[n]["value"]) if matched
+            else "jacobi")
+    views = runs[best]["seeded_views"]
+    # the network alone as the chosen setup runs it: one pass over the
+    # padded pair (and its mirror, as a batch of two, when both views are
+    # seeded); best of five after a warm-up
     Hp, Wp = padded(H, W)
-    x = torch.rand((2, Hp, Wp, 8), device=dL.device)
-    out = torch.empty((2, Hp, Wp, 4), device=dL.device)
+    x = torch.rand((views, Hp, Wp, 8), device=dL.device)
+    out = torch.empty((views, Hp, Wp, 4), device=dL.device)
     fwd = []
     with torch.cuda.stream(stream):
         for _ in range(6):
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            net.forward(x.data_ptr(), 2, Hp, Wp, out.data_ptr())
+            net.forward(x.data_ptr(), views, Hp, Wp, out.data_ptr())
             e1.record(stream)
             torch.cuda.synchronize()
             fwd.append(e0.elapsed_time(e1))
     fwd_ms = float(np.min(fwd[1:]))
     net.close()
-    flops = forward_flops(H, W, N=2)
+    flops = forward_flops(H, W, N=views)
     tflops = flops / (fwd_ms * 1e-3) / 1e12
-    matched = [n for n in runs if runs[n]["matched"]]
-    best = (max(matched, key=lambda n: runs[n]["value"]) if matched
-            else "jacobi")
     out = dict(runs[best])
     out.update({
         "schedule": best, "schedules": runs,
@@ -384,8 +392,9 @@ def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
                          "frac": tflops / BF16_PEAK_TFLOPS,
                          "traffic": None,
                          "forward_ms": fwd_ms, "flop_per_forward": flops,
-                         "forward": "one pass over the pair and its mirror "
-                                    "(batch of two)"}})
+                         "forward": ("one pass over the pair and its mirror "
+                                     "(batch of two)" if views == 2 else
+                                     "one pass over the pair")}})
     return out
 
 
@@ -499,10 +508,11 @@ def serving_section(stream, cost_kind, cost_weight, runs=3):
     """Batched learned serving mix: 16 RGBA pairs from the GPU generator,
     width uniform in 320-1920, height in 240-1080, disparity range 0.1 x
     width (up to 192 px); plain 8-iteration batched PatchMatch on them,
-    then the proposal network seeding every pair and k refine iterations,
-    k the fewest whose pooled bad-1.0px (from the on-device error counts)
-    is at most plain's; each run one graph launch.  None without a
-    trained weights file."""
+    then the proposal network seeding both views of every pair, or the
+    left views only, and k refine iterations, k the fewest whose pooled
+    bad-1.0px (from the on-device error counts) is at most plain's; the
+    faster matched seeding is reported; each run one graph launch.  None
+    without a trained weights file."""
     import numpy as np
     import torch
 
@@ -553,27 +563,38 @@ def serving_section(stream, cost_kind, cost_weight, runs=3):
 
         b.set_proposal(None)
         out["plain"] = timed(ITERS)
-        # learned: bad-1px after k = 0, 1, ... refine iterations until it
-        # reaches plain's
-        b.set_proposal(net)
-        bad = []
-        for k in range(ITERS + 1):
-            b.run(k, pp, MAXDISP, MAXDEG, 1.0, 0.0)
-            bad.append(pooled_bad()[0])
-            if matched_iterations(bad, out["plain"]["bad_1px_pooled"]):
-                break
-        matched = matched_iterations(bad, out["plain"]["bad_1px_pooled"])
-        out["learned"] = timed(matched if matched is not None else ITERS)
-        out["learned"]["matched"] = matched is not None
-        out["learned"]["bad_1px_pooled_by_refine_iterations"] = bad
+        # learned, both views or the left view seeded: bad-1px after k = 0,
+        # 1, ... refine iterations until it reaches plain's
+        target = out["plain"]["bad_1px_pooled"]
+        seeded = {}
+        for views in (2, 1):
+            b.set_proposal(net, views)
+            bad = []
+            for k in range(ITERS + 1):
+                b.run(k, pp, MAXDISP, MAXDEG, 1.0, 0.0)
+                bad.append(pooled_bad()[0])
+                if matched_iterations(bad, target):
+                    break
+            matched = matched_iterations(bad, target)
+            r = timed(matched if matched is not None else ITERS)
+            r["matched"] = matched is not None
+            r["seeded_views"] = views
+            r["bad_1px_pooled_by_refine_iterations"] = bad
+            seeded[views] = r
+        ok = [v for v in seeded if seeded[v]["matched"]] or list(seeded)
+        out["learned"] = dict(seeded[max(ok, key=lambda v:
+                                         seeded[v]["value"])])
+        out["learned"]["other_seeding"] = seeded[
+            1 if out["learned"]["seeded_views"] == 2 else 2]
         tuning = b.tuning
     return {"workload": "batched serving mix: 16 RGBA f32 pairs, width "
                         "320-1920, height 240-1080 (seeded), 24 cells, "
                         "disparity 0.1 x width (<= 192 px), 9x9 patch, LR "
-                        "check 1 px; learned: proposal network + the "
-                        "fewest refine iterations whose pooled bad-1px is at "
-                        "most plain's, plain: 8 iterations; one graph launch "
-                        "per batch",
+                        "check 1 px; learned: proposal network seeding both "
+                        "views or the left view (the faster matched one) + "
+                        "the fewest refine iterations whose pooled bad-1px "
+                        "is at most plain's, plain: 8 iterations; one graph "
+                        "launch per batch",
             "learned": out["learned"], "plain": out["plain"],
             "speedup_vs_plain": out["learned"]["value"] /
             out["plain"]["value"],
@@ -825,17 +846,24 @@ def main():
             result["ms_per_step"] = learned["ms_per_step"]
             result["config"]["variant"] = "learned"
             result["config"]["refine_iterations"] = k
-            result["config"]["refine_schedule"] = learned["schedule"]
+            result["config"]["refine_schedule"] = (
+                "red_black" if learned["schedule"].startswith("red")
+                else "jacobi")
+            result["config"]["seeded_views"] = learned["seeded_views"]
+            seeding = ("on the pair and its mirror) seeding both views"
+                       if learned["seeded_views"] == 2 else
+                       "on the pair) seeding the left view, the right view "
+                       "from the random init,")
+            sched = ("red-black" if learned["schedule"].startswith("red")
+                     else "jacobi")
             result["config"]["workload"] = (
                 "learned-init PatchMatch: bf16 proposal network (4-level "
-                "32-64-96-128 U-Net on the pair and its mirror) seeding both "
-                "views with per-pixel fallback to random init, then %d "
-                "refine iteration%s (%s schedule); 1280x720 RGBA f32 pair, "
-                "24 planar "
-                "cells, disparity 0-128 px, 9x9 patch, %s, LR check 1 px; "
-                "against plain PatchMatch with 8 iterations"
-                % (k, "" if k == 1 else "s",
-                   learned["schedule"].replace("_", "-"),
+                "32-64-96-128 U-Net %s with per-pixel fallback to random "
+                "init, then %d refine iteration%s (%s schedule); 1280x720 "
+                "RGBA f32 pair, 24 planar cells, disparity 0-128 px, 9x9 "
+                "patch, %s, LR check 1 px; against plain PatchMatch with 8 "
+                "iterations"
+                % (seeding, k, "" if k == 1 else "s", sched,
                    "gamma-10 truncated colour+gradient cost (weight 25.5)"
                    if args.cost == "color_grad" else "census cost (weight 12)"))
             result["quality"] = {"bad_1px": learned["bad_1px"],

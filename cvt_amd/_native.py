@@ -91,6 +91,7 @@ SIGNATURES = {
     "pm_net_train_step": (_i, [_p, _p, _p, _i, _i, _i, _f, _f, _f,
                                ctypes.POINTER(_f)]),
     "pm_setup_proposal": (_i, [_p, _p]),
+    "pm_setup_proposal_views": (_i, [_p, _p, _i]),
     # include/pm_batch.h
     "pm_batch_create": (_i, [ctypes.POINTER(PmParams), _p, _i,
                              ctypes.POINTER(_p)]),
@@ -107,6 +108,7 @@ SIGNATURES = {
     "pm_batch_get_tuning": (_i, [_p, _p]),
     "pm_batch_set_tuning": (_i, [_p, _p]),
     "pm_batch_set_proposal": (_i, [_p, _p]),
+    "pm_batch_set_proposal_views": (_i, [_p, _p, _i]),
     "pm_batch_sync": (_i, [_p]),
     # include/pm_io.h
     "pm_io_last_error": (ctypes.c_char_p, []),

@@ -144,13 +144,13 @@ class Batch:
                           workgroups=int(workgroups), ms=0.0)
         self._check(self._lib.pm_batch_set_tuning(self._h, ctypes.byref(t)))
 
-    def set_proposal(self, net):
+    def set_proposal(self, net, views=2):
         """Seed every pair from a ProposalNet (cvt_amd.net) in each run's
-        setup; None restores the random init.  The net must outlive its use
-        by the batch."""
+        setup (views=1: the left views only); None restores the random
+        init.  The net must outlive its use by the batch."""
         self._net = net
-        self._check(self._lib.pm_batch_set_proposal(
-            self._h, net.handle if net is not None else None))
+        self._check(self._lib.pm_batch_set_proposal_views(
+            self._h, net.handle if net is not None else None, int(views)))
 
     def sync(self):
         self._check(self._lib.pm_batch_sync(self._h))

@@ -68,6 +68,7 @@ struct pm_batch {
   bool counted = false;              // the last run counted errors
 
   pm_net* net = nullptr;              // proposal seeding (may be null)
+  int net_views = 2;                  // views the proposal seeds (1 or 2)
 
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -170,25 +171,30 @@ size_t tile_lds(const pm_batch* b, int rows) {
   return pm::batch_lds_bytes(b->prm.radius, b->prm.cost_kind, rows);
 }
 
-// the proposal-seeded init of every pair (pm_setup_proposal): the
-// network's pass over pair i and its mirror, then the init of both views,
-// in stream order (the next pass overwrites the network's output)
-int enqueue_proposal_init(pm_batch* b, hipStream_t st) {
+// the proposal-seeded init of every pair (pm_setup_proposal_views): the
+// network's pass over pair i (and its mirror), then the init of the seeded
+// views, in stream order (the next pass overwrites the network's output);
+// with one seeded view the right views take the random init afterwards
+int enqueue_proposal_init(pm_batch* b, hipStream_t st, size_t lds) {
   for (int i = 0; i < b->n; ++i) {
     const pm::BatchPair& P = b->host[i];
     const float* prop[2] = {nullptr, nullptr};
     int Wp = 0, os = 0;
     if (pm::net::proposal(b->net, st, P.img[0], P.img[1], P.g.W, P.g.H,
-                          P.channels, P.stride, prop, &Wp, &os))
+                          P.channels, P.stride, b->net_views, prop, &Wp,
+                          &os))
       return fail(b, PM_ERR_HIP, "proposal network, pair %d: %s", i,
                   pm::net::error_of(b->net));
-    for (int view = 0; view < 2; ++view)
+    for (int view = 0; view < b->net_views; ++view)
       PB_HIP(b, pm::launch_init_proposal(st, P.g, P.v[view], P.v[1 - view],
                                          P.dmax, view == 0 ? 1 : 0,
                                          prop[view], os, Wp,
                                          PM_NET_DISP_SCALE,
                                          P.state[view][0]));
   }
+  if (b->net_views == 1)
+    PB_HIP(b, pm::launch_init_batch(st, b->dev, b->n, b->nitems, 1,
+                                    b->prm.cost_kind, lds, b->shape));
   return PM_OK;
 }
 
@@ -201,7 +207,7 @@ int enqueue_setup(pm_batch* b, hipStream_t st) {
                                      b->prm.radius, b->prm.color_weight));
   if (b->net) {
     int rc;
-    if ((rc = enqueue_proposal_init(b, st))) return rc;
+    if ((rc = enqueue_proposal_init(b, st, lds))) return rc;
   } else {
     for (int view = 0; view < 2; ++view)
       PB_HIP(b, pm::launch_init_batch(st, b->dev, b->n, b->nitems, view,
@@ -646,7 +652,13 @@ int pm_batch_autotune(pm_batch* b, int reps, pm_batch_tuning* table,
 }
 
 int pm_batch_set_proposal(pm_batch* b, pm_net* net) {
+  return pm_batch_set_proposal_views(b, net, 2);
+}
+
+int pm_batch_set_proposal_views(pm_batch* b, pm_net* net, int views) {
   if (!b) return PM_ERR_ARG;
+  if (views != 1 && views != 2)
+    return fail(b, PM_ERR_ARG, "views must be 1 or 2, not %d", views);
   PB_HIP(b, hipSetDevice(b->prm.device));
   if (net) {
     if (pm::net::device_of(net) != b->prm.device)
@@ -655,8 +667,9 @@ int pm_batch_set_proposal(pm_batch* b, pm_net* net) {
     int rc;
     if ((rc = reserve_net(b, net))) return rc;
   }
-  if (net != b->net) {
+  if (net != b->net || views != b->net_views) {
     b->net = net;
+    b->net_views = views;
     drop_graph(b);
     b->ran = false;
   }

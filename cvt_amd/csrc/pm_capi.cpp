@@ -864,8 +864,14 @@ int pm_setup(pm_ctx* c) {
 }
 
 int pm_setup_proposal(pm_ctx* c, pm_net* net) {
+  return pm_setup_proposal_views(c, net, 2);
+}
+
+int pm_setup_proposal_views(pm_ctx* c, pm_net* net, int views) {
   if (!c) return PM_ERR_ARG;
   if (!net) return fail(c, PM_ERR_ARG, "net is NULL");
+  if (views != 1 && views != 2)
+    return fail(c, PM_ERR_ARG, "views must be 1 or 2, not %d", views);
   int rc;
   if ((rc = require_images(c)) || (rc = use_device(c))) return rc;
   if (pm::net::device_of(net) != c->prm.device)
@@ -875,19 +881,25 @@ int pm_setup_proposal(pm_ctx* c, pm_net* net) {
   const float* img[2] = {static_cast<const float*>(c->img[0].ptr),
                          static_cast<const float*>(c->img[1].ptr)};
   const int W = c->geo.W, H = c->geo.H, C = c->channels;
-  // the network on the pair (left view) and the mirrored pair (right view)
+  // the network on the pair (left view) and, with views = 2, on the
+  // mirrored pair (right view); with views = 1 the right view takes the
+  // random init of pm_setup
   const float* prop[2] = {nullptr, nullptr};
   int Wp = 0, os = 0;
-  if (pm::net::proposal(net, c->stream, img[0], img[1], W, H, C, W * C, prop,
-                        &Wp, &os))
+  if (pm::net::proposal(net, c->stream, img[0], img[1], W, H, C, W * C, views,
+                        prop, &Wp, &os))
     return fail(c, PM_ERR_HIP, "proposal network: %s",
                 pm::net::error_of(net));
-  for (int view = 0; view < 2; ++view)
+  for (int view = 0; view < views; ++view)
     PM_HIP(c, pm::launch_init_proposal(
                   c->stream, c->geo, view_bufs(c, view),
                   view_bufs(c, 1 - view), c->prm.disparity_max,
                   view == 0 ? 1 : 0, prop[view], os, Wp, PM_NET_DISP_SCALE,
                   state_ptr(c, view, 0)));
+  if (views == 1)
+    PM_HIP(c, pm::launch_init(c->stream, c->geo, view_bufs(c, 1),
+                              view_bufs(c, 0), c->prm.disparity_max, 0,
+                              state_ptr(c, 1, 0)));
   PM_HIP(c, pm::launch_viewbuf_clear(
                 c->stream, c->geo,
                 static_cast<unsigned long long*>(c->vbuf[1].ptr)));

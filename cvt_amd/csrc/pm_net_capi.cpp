@@ -310,23 +310,25 @@ namespace pm {
 namespace net {
 
 int proposal(pm_net* n, hipStream_t st, const float* left, const float* right,
-             int W, int H, int channels, int stride, const float** out,
-             int* Wp, int* os) {
+             int W, int H, int channels, int stride, int views,
+             const float** out, int* Wp, int* os) {
+  if (views != 1 && views != 2)
+    return fail(n, PM_ERR_ARG, "views must be 1 or 2, not %d", views);
   const int Hp = (H + 7) / 8 * 8, Wq = (W + 7) / 8 * 8;
   int rc;
-  if ((rc = alloc_batch(n, 2, Hp, Wq))) return rc;
+  if ((rc = alloc_batch(n, views, Hp, Wq))) return rc;
   // batch entry 0: the pair; entry 1: the mirrored pair.  Every output
   // pixel depends on its own batch entry only, so each entry's result is
   // what a pass over it alone gives, bit for bit.
   const size_t plane = (size_t)Hp * Wq;
-  for (int mirror = 0; mirror < 2; ++mirror)
+  for (int mirror = 0; mirror < views; ++mirror)
     NET_HIP(n, launch_pack_pair(st, left, right, W, H, channels, stride, Hp,
                                 Wq, mirror,
                                 as<bf16>(n->act[X0]) +
                                     mirror * plane * kT[X0].ch));
   if ((rc = forward(n, st))) return rc;
   out[0] = as<float>(n->act[OUT]);
-  out[1] = out[0] + plane * kT[OUT].ch;
+  out[1] = views == 2 ? out[0] + plane * kT[OUT].ch : nullptr;
   *Wp = Wq;
   *os = kT[OUT].ch;
   return PM_OK;

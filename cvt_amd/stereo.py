@@ -147,10 +147,13 @@ class Context:
                                              _ptr(n) if n is not None
                                              else None))
 
-    def setup_proposal(self, net):
+    def setup_proposal(self, net, views=2):
         """Setup seeded by a plane-proposal network (cvt_amd.net.ProposalNet)
-        with a per-pixel fallback to the random init (pm_setup_proposal)."""
-        self._check(self._lib.pm_setup_proposal(self._h, net.handle))
+        with a per-pixel fallback to the random init (pm_setup_proposal);
+        views=1 seeds the left view only, the right view taking the random
+        init (pm_setup_proposal_views)."""
+        self._check(self._lib.pm_setup_proposal_views(self._h, net.handle,
+                                                      int(views)))
 
     # -- iterate -------------------------------------------------------------
     def iterate(self, n=1):

@@ -87,6 +87,14 @@ int pm_net_get_grads(pm_net* net, float* out, size_t n);
  * stream. */
 int pm_setup_proposal(pm_ctx* ctx, pm_net* net);
 
+/* pm_setup_proposal with a choice of seeded views (no counterpart in the
+ * reference): views = 2 is pm_setup_proposal; views = 1 runs the network
+ * on the pair only, seeds the left view as above and gives the right view
+ * the random init of pm_setup (pmhstereo_init).  The first iteration's
+ * view propagation then carries the left view's planes over to the right
+ * view, and the network's pass over the mirrored pair is saved. */
+int pm_setup_proposal_views(pm_ctx* ctx, pm_net* net, int views);
+
 #ifdef __cplusplus
 }
 #endif

@@ -201,6 +201,8 @@ int main(void) {
     if (pm_net_create(0, NULL, 1, &net) != PM_ERR_HIP) return 3;
     pm_net_destroy(net);
     if (pm_setup_proposal(NULL, NULL) != PM_ERR_ARG) return 4;
+    if (pm_setup_proposal_views(NULL, NULL, 1) != PM_ERR_ARG) return 4;
+    if (pm_batch_set_proposal_views(NULL, NULL, 1) != PM_ERR_ARG) return 4;
     {   /* a batch checks its arguments before it looks for a device */
         pm_batch* b = 0;
         pm_batch_pair q = {16, 8, 4, 64, 4.0f, (const float*)&p,

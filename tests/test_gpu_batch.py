@@ -187,10 +187,12 @@ def test_set_pair_swaps_images_without_recording(inputs, synth):
             b.set_pair(0, dict(batch_pairs([q2])[0], width=q0["W"] + 1))
 
 
-def test_proposal_seeded_batch_equals_single_pairs(inputs):
-    """pm_batch_set_proposal: every pair seeded as pm_setup_proposal seeds a
-    context (an untrained network: the seeding, not its quality, is under
-    test), then two iterations, inside the recorded graph"""
+@pytest.mark.parametrize("views", [2, 1])
+def test_proposal_seeded_batch_equals_single_pairs(inputs, views):
+    """pm_batch_set_proposal_views: every pair seeded as
+    pm_setup_proposal_views seeds a context (an untrained network: the
+    seeding, not its quality, is under test), then two iterations, inside
+    the recorded graph"""
     from cvt_amd.net import ProposalNet
     kind, weight = COSTS[1]
     with ProposalNet(seed=5) as net:
@@ -199,14 +201,14 @@ def test_proposal_seeded_batch_equals_single_pairs(inputs):
             with cvt_amd.Context(radius=RADIUS, disparity_max=q["dmax"],
                                  cost_kind=kind, cost_weight=weight) as ctx:
                 ctx.set_images(q["L"], q["R"])
-                ctx.setup_proposal(net)
+                ctx.setup_proposal(net, views)
                 ctx.iterate(2)
                 want.append((ctx.planes(0), ctx.planes(1),
                              ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0,
                                            5.0)))
         with Batch(batch_pairs(inputs), radius=RADIUS, cost_kind=kind,
                    cost_weight=weight) as b:
-            b.set_proposal(net)
+            b.set_proposal(net, views)
             for replay in range(2):
                 b.run(2, cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 5.0)
                 check_batch(b, inputs, want, ("proposal", replay))

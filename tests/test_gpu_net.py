@@ -221,10 +221,12 @@ def test_training_lowers_the_loss():
     assert np.mean(losses[-5:]) < 0.7 * np.mean(losses[:3]), losses
 
 
-def test_setup_proposal_bitexact(oracle, synth):
-    """pm_setup_proposal against the oracle: the proposal (the network's
-    output on the padded pair, and on the mirrored pair for the right view)
-    competes with the random init plane at every pixel."""
+@pytest.mark.parametrize("views", [2, 1])
+def test_setup_proposal_bitexact(oracle, synth, views):
+    """pm_setup_proposal(_views) against the oracle: the proposal (the
+    network's output on the padded pair, and on the mirrored pair for the
+    right view) competes with the random init plane at every pixel; with
+    views = 1 the right view is the random init alone."""
     import torch
     L, R, _ = synth.make_pair(61, 37, cells=4, dmax=16.0, channels=4, seed=19)
     H, W = L.shape[:2]
@@ -250,11 +252,14 @@ def test_setup_proposal_bitexact(oracle, synth):
         prop_r = run(R[:, ::-1], L[:, ::-1])
         with cvt_amd.Context(radius=r, disparity_max=dmax) as ctx:
             ctx.set_images(L, R)
-            ctx.setup_proposal(net)
+            ctx.setup_proposal(net, views)
             got = (ctx.planes(0), ctx.planes(1))
     tl, tr = oracle.prepare(L, r), oracle.prepare(R, r)
     for view, (a, b, lr) in enumerate(((tl, tr, 1), (tr, tl, 0))):
         rnd = oracle.init(a, b, r, dmax, lr)
+        if view >= views:                 # not seeded: the random init
+            assert same(got[view], rnd), (view, first_diff(got[view], rnd))
+            continue
         prop = np.zeros_like(rnd)
         have = np.zeros((H, W), bool)
         for y in range(H):
