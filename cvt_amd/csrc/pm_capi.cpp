@@ -83,6 +83,9 @@ struct pm_ctx {
   hipStream_t smoothing = nullptr;
   hipEvent_t ev_band[2][kSplitParts] = {};
   hipEvent_t ev_smooth[2] = {};
+  // pm_setup_proposal_views: the texel work forks onto side[0] beside the
+  // network's pass (ev_setup[0]) and joins back (ev_setup[1])
+  hipEvent_t ev_setup[2] = {};
   // LDS the propagate's work-groups ask for at least (pm_create)
   size_t prop_lds_floor = 0;
 };
@@ -287,9 +290,9 @@ int set_images(pm_ctx* c, const float* l, const float* r, int W, int H,
 }
 
 // bilateral weights + gradients of both views (PMHuberStereo.cpp:445-450)
-int prepare_views(pm_ctx* c) {
+int prepare_views(pm_ctx* c, hipStream_t st = nullptr) {
   for (int v = 0; v < 2; ++v)
-    PM_HIP(c, pm::launch_prepare(c->stream, c->geo,
+    PM_HIP(c, pm::launch_prepare(st ? st : c->stream, c->geo,
                                  static_cast<const float*>(c->img[v].ptr),
                                  c->channels, c->geo.W * c->channels,
                                  view_bufs(c, v)));
@@ -822,6 +825,7 @@ void pm_destroy(pm_ctx* c) {
     for (hipEvent_t e : c->ev_band[v])
       if (e) (void)hipEventDestroy(e);
     if (c->ev_smooth[v]) (void)hipEventDestroy(c->ev_smooth[v]);
+    if (c->ev_setup[v]) (void)hipEventDestroy(c->ev_setup[v]);
   }
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -877,29 +881,50 @@ int pm_setup_proposal_views(pm_ctx* c, pm_net* net, int views) {
   if (pm::net::device_of(net) != c->prm.device)
     return fail(c, PM_ERR_ARG, "the net is on device %d, the context on %d",
                 pm::net::device_of(net), c->prm.device);
-  if ((rc = prepare_views(c))) return rc;
+  // The texels of both views (and, with views = 1, the right view's
+  // random init of pm_setup) on a side stream, beside the network's pass
+  // on the context's stream: neither reads what the other writes, and the
+  // network's deeper layers leave the GPU room.  The seeding waits for
+  // both.  Everything queued on the context's stream before comes first.
+  if (!c->side[0])
+    PM_HIP(c, hipStreamCreateWithFlags(&c->side[0], hipStreamNonBlocking));
+  for (hipEvent_t& e : c->ev_setup)
+    if (!e) PM_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const hipStream_t side = c->side[0];
+  PM_HIP(c, hipEventRecord(c->ev_setup[0], c->stream));
+  PM_HIP(c, hipStreamWaitEvent(side, c->ev_setup[0], 0));
+  int side_rc = prepare_views(c, side);
+  if (!side_rc && views == 1) {
+    const hipError_t e = pm::launch_init(side, c->geo, view_bufs(c, 1),
+                                         view_bufs(c, 0),
+                                         c->prm.disparity_max, 0,
+                                         state_ptr(c, 1, 0));
+    if (e != hipSuccess) side_rc = hip_fail(c, e, "init of the right view");
+  }
   const float* img[2] = {static_cast<const float*>(c->img[0].ptr),
                          static_cast<const float*>(c->img[1].ptr)};
   const int W = c->geo.W, H = c->geo.H, C = c->channels;
   // the network on the pair (left view) and, with views = 2, on the
-  // mirrored pair (right view); with views = 1 the right view takes the
-  // random init of pm_setup
+  // mirrored pair (right view)
   const float* prop[2] = {nullptr, nullptr};
   int Wp = 0, os = 0;
-  if (pm::net::proposal(net, c->stream, img[0], img[1], W, H, C, W * C, views,
-                        prop, &Wp, &os))
+  rc = side_rc ? side_rc
+               : pm::net::proposal(net, c->stream, img[0], img[1], W, H, C,
+                                   W * C, views, prop, &Wp, &os);
+  // joined whatever failed, so that no side work is left unordered
+  hipError_t e = hipEventRecord(c->ev_setup[1], side);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->ev_setup[1], 0);
+  if (side_rc) return side_rc;
+  if (rc)
     return fail(c, PM_ERR_HIP, "proposal network: %s",
                 pm::net::error_of(net));
+  if (e != hipSuccess) return hip_fail(c, e, "join of the setup streams");
   for (int view = 0; view < views; ++view)
     PM_HIP(c, pm::launch_init_proposal(
                   c->stream, c->geo, view_bufs(c, view),
                   view_bufs(c, 1 - view), c->prm.disparity_max,
                   view == 0 ? 1 : 0, prop[view], os, Wp, PM_NET_DISP_SCALE,
                   state_ptr(c, view, 0)));
-  if (views == 1)
-    PM_HIP(c, pm::launch_init(c->stream, c->geo, view_bufs(c, 1),
-                              view_bufs(c, 0), c->prm.disparity_max, 0,
-                              state_ptr(c, 1, 0)));
   PM_HIP(c, pm::launch_viewbuf_clear(
                 c->stream, c->geo,
                 static_cast<unsigned long long*>(c->vbuf[1].ptr)));
