@@ -174,8 +174,10 @@ size_t tile_lds(const pm_batch* b, int rows) {
 // the proposal-seeded init of every pair (pm_setup_proposal_views): the
 // network's pass over pair i (and its mirror), then the init of the seeded
 // views, in stream order (the next pass overwrites the network's output);
-// with one seeded view the right views take the random init afterwards
-int enqueue_proposal_init(pm_batch* b, hipStream_t st, size_t lds) {
+// with one seeded view the right view follows from the left through the
+// view scatter (into the left view's slots, cleared again by its first
+// propagate)
+int enqueue_proposal_init(pm_batch* b, hipStream_t st) {
   for (int i = 0; i < b->n; ++i) {
     const pm::BatchPair& P = b->host[i];
     const float* prop[2] = {nullptr, nullptr};
@@ -191,10 +193,13 @@ int enqueue_proposal_init(pm_batch* b, hipStream_t st, size_t lds) {
                                          prop[view], os, Wp,
                                          PM_NET_DISP_SCALE,
                                          P.state[view][0]));
+    if (b->net_views == 1) {
+      PB_HIP(b, pm::launch_viewbuf_clear(st, P.g, P.vbuf[0]));
+      PB_HIP(b, pm::launch_scatter(st, P.g, P.state[0][0], 1, P.vbuf[0]));
+      PB_HIP(b, pm::launch_init_from_view(st, P.g, P.dmax, P.vbuf[0],
+                                          P.state[0][0], P.state[1][0]));
+    }
   }
-  if (b->net_views == 1)
-    PB_HIP(b, pm::launch_init_batch(st, b->dev, b->n, b->nitems, 1,
-                                    b->prm.cost_kind, lds, b->shape));
   return PM_OK;
 }
 
@@ -207,7 +212,7 @@ int enqueue_setup(pm_batch* b, hipStream_t st) {
                                      b->prm.radius, b->prm.color_weight));
   if (b->net) {
     int rc;
-    if ((rc = enqueue_proposal_init(b, st, lds))) return rc;
+    if ((rc = enqueue_proposal_init(b, st))) return rc;
   } else {
     for (int view = 0; view < 2; ++view)
       PB_HIP(b, pm::launch_init_batch(st, b->dev, b->n, b->nitems, view,

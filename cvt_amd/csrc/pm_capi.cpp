@@ -881,8 +881,7 @@ int pm_setup_proposal_views(pm_ctx* c, pm_net* net, int views) {
   if (pm::net::device_of(net) != c->prm.device)
     return fail(c, PM_ERR_ARG, "the net is on device %d, the context on %d",
                 pm::net::device_of(net), c->prm.device);
-  // The texels of both views (and, with views = 1, the right view's
-  // random init of pm_setup) on a side stream, beside the network's pass
+  // The texels of both views on a side stream, beside the network's pass
   // on the context's stream: neither reads what the other writes, and the
   // network's deeper layers leave the GPU room.  The seeding waits for
   // both.  Everything queued on the context's stream before comes first.
@@ -893,14 +892,7 @@ int pm_setup_proposal_views(pm_ctx* c, pm_net* net, int views) {
   const hipStream_t side = c->side[0];
   PM_HIP(c, hipEventRecord(c->ev_setup[0], c->stream));
   PM_HIP(c, hipStreamWaitEvent(side, c->ev_setup[0], 0));
-  int side_rc = prepare_views(c, side);
-  if (!side_rc && views == 1) {
-    const hipError_t e = pm::launch_init(side, c->geo, view_bufs(c, 1),
-                                         view_bufs(c, 0),
-                                         c->prm.disparity_max, 0,
-                                         state_ptr(c, 1, 0));
-    if (e != hipSuccess) side_rc = hip_fail(c, e, "init of the right view");
-  }
+  const int side_rc = prepare_views(c, side);
   const float* img[2] = {static_cast<const float*>(c->img[0].ptr),
                          static_cast<const float*>(c->img[1].ptr)};
   const int W = c->geo.W, H = c->geo.H, C = c->channels;
@@ -925,6 +917,18 @@ int pm_setup_proposal_views(pm_ctx* c, pm_net* net, int views) {
                   view_bufs(c, 1 - view), c->prm.disparity_max,
                   view == 0 ? 1 : 0, prop[view], os, Wp, PM_NET_DISP_SCALE,
                   state_ptr(c, view, 0)));
+  if (views == 1) {
+    // the right view from the seeded left view through the view scatter
+    // (into the left view's slots, which its first propagate clears)
+    unsigned long long* vbL = static_cast<unsigned long long*>(c->vbuf[0].ptr);
+    PM_HIP(c, pm::launch_viewbuf_clear(c->stream, c->geo, vbL));
+    PM_HIP(c, pm::launch_scatter(c->stream, c->geo, state_ptr(c, 0, 0), 1,
+                                 vbL));
+    PM_HIP(c, pm::launch_init_from_view(c->stream, c->geo,
+                                        c->prm.disparity_max, vbL,
+                                        state_ptr(c, 0, 0),
+                                        state_ptr(c, 1, 0)));
+  }
   PM_HIP(c, pm::launch_viewbuf_clear(
                 c->stream, c->geo,
                 static_cast<unsigned long long*>(c->vbuf[1].ptr)));

@@ -118,6 +118,14 @@ hipError_t launch_scatter(hipStream_t st, const Geometry& g,
                           const float4* states, int lr,
                           unsigned long long* viewout);
 
+// right-view planes (lr = 0) from the left view's field src through the
+// slots a left-view scatter (lr = 1) filled: the nearest surface where one
+// landed, pmhstereo_init's random plane elsewhere; cost channel NaN
+hipError_t launch_init_from_view(hipStream_t st, const Geometry& g,
+                                 float dmax,
+                                 const unsigned long long* slots,
+                                 const float4* src, float4* states);
+
 hipError_t launch_cost_field(hipStream_t st, const Geometry& g, ViewBufs self,
                              ViewBufs other, const float4* planes,
                              float* cost);

@@ -715,6 +715,36 @@ k_init_proposal(Geometry g, ViewBufs self, ViewBufs other, float dmax, int lr,
   }
 }
 
+// The right view's start when only the left view is seeded
+// (pm_setup_proposal_views, views = 1): at every pixel the nearest left
+// surface the view scatter put there (slot key 0, the propagate's first
+// view candidate, pmhstereo.cl:1007-1016) carried over by viewprop, else
+// the random plane of pmhstereo_init (draws 3p..3p+2, pmhstereo.cl:
+// 755-773).  No cost is evaluated: the cost channel is NaN, and every
+// propagate re-costs its own planes before it uses them
+// (pmhstereo.cl:940).
+__global__ void __launch_bounds__(256)
+k_init_from_view(int W, int H, float dmax,
+                 const unsigned long long* __restrict__ slots,
+                 const float4* __restrict__ src, float4* __restrict__ states) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  const size_t p = (size_t)y * W + x;
+  const uint32_t k0 = (uint32_t)(slots[p] & 0xFFFFFFFFull);
+  float4 s;
+  if (k0 != kSlotEmpty) {
+    // the left view's key is W - 1 - x (slot_insert's order: nearest first)
+    s = viewprop(src[(size_t)y * W + (W - 1 - (int)k0)]);
+  } else {
+    Mwc rng = mwc_jump((uint64_t)p * 3u);
+    const float u0 = mwc_next(&rng), u1 = mwc_next(&rng), u2 = mwc_next(&rng);
+    s = state_init(u0, u1, u2, (float)x, (float)y, 0, dmax);
+  }
+  s.w = __int_as_float(0x7fc00000);
+  states[p] = s;
+}
+
 __global__ void __launch_bounds__(256)
 k_viewbuf_clear(size_t n, unsigned long long* __restrict__ vbuf) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -1425,6 +1455,15 @@ hipError_t launch_propagate(hipStream_t st, const Geometry& g,
                                                       lds_floor)
             : launch_propagate_kind<kCostCensus, false>(st, g, a, counters,
                                                        lds_floor);
+}
+
+hipError_t launch_init_from_view(hipStream_t st, const Geometry& g,
+                                 float dmax,
+                                 const unsigned long long* slots,
+                                 const float4* src, float4* states) {
+  PM_LAUNCH(k_init_from_view,
+            dim3((g.W + kBX - 1) / kBX, (g.H + kBY - 1) / kBY), kBlock2D, 0,
+            st, g.W, g.H, dmax, slots, src, states);
 }
 
 hipError_t launch_scatter(hipStream_t st, const Geometry& g,

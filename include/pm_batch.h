@@ -118,8 +118,9 @@ int pm_batch_set_proposal(pm_batch* b, pm_net* net);
 
 /* pm_batch_set_proposal with a choice of seeded views, as
  * pm_setup_proposal_views (include/pm_net.h): views = 1 seeds only the
- * left view of every pair (the network on the pair alone) and gives the
- * right view the random init; views = 2 is pm_batch_set_proposal. */
+ * left view of every pair (the network on the pair alone) and carries it
+ * over to the right view through the view scatter; views = 2 is
+ * pm_batch_set_proposal. */
 int pm_batch_set_proposal_views(pm_batch* b, pm_net* net, int views);
 
 int pm_batch_sync(pm_batch* b);

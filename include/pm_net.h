@@ -89,10 +89,12 @@ int pm_setup_proposal(pm_ctx* ctx, pm_net* net);
 
 /* pm_setup_proposal with a choice of seeded views (no counterpart in the
  * reference): views = 2 is pm_setup_proposal; views = 1 runs the network
- * on the pair only, seeds the left view as above and gives the right view
- * the random init of pm_setup (pmhstereo_init).  The first iteration's
- * view propagation then carries the left view's planes over to the right
- * view, and the network's pass over the mirrored pair is saved. */
+ * on the pair only and seeds the left view as above; the right view takes,
+ * at every pixel, the nearest left surface that the view scatter of the
+ * seeded left field puts there (the plane the propagate's first view
+ * candidate would read), and pmhstereo_init's random plane where none
+ * lands, with no cost evaluated (NaN: every propagate re-costs its planes
+ * first).  The network's pass over the mirrored pair is saved. */
 int pm_setup_proposal_views(pm_ctx* ctx, pm_net* net, int views);
 
 #ifdef __cplusplus

@@ -226,7 +226,9 @@ def test_setup_proposal_bitexact(oracle, synth, views):
     """pm_setup_proposal(_views) against the oracle: the proposal (the
     network's output on the padded pair, and on the mirrored pair for the
     right view) competes with the random init plane at every pixel; with
-    views = 1 the right view is the random init alone."""
+    views = 1 the right view takes, through the view scatter of the seeded
+    left view, the nearest left surface at each pixel (viewprop), the
+    random init plane where none lands, and no cost (NaN)."""
     import torch
     L, R, _ = synth.make_pair(61, 37, cells=4, dmax=16.0, channels=4, seed=19)
     H, W = L.shape[:2]
@@ -257,8 +259,17 @@ def test_setup_proposal_bitexact(oracle, synth, views):
     tl, tr = oracle.prepare(L, r), oracle.prepare(R, r)
     for view, (a, b, lr) in enumerate(((tl, tr, 1), (tr, tl, 0))):
         rnd = oracle.init(a, b, r, dmax, lr)
-        if view >= views:                 # not seeded: the random init
-            assert same(got[view], rnd), (view, first_diff(got[view], rnd))
+        if view >= views:
+            # from the left view's planes (checked above) by the scatter
+            left = got[0]
+            key0 = (oracle.scatter(left, 1) & 0xFFFFFFFF).astype(np.int64)
+            want = rnd.copy()
+            landed = key0 != 0xFFFFFFFF
+            for y, x in zip(*np.nonzero(landed)):
+                want[y, x] = oracle.viewprop(left[y, W - 1 - key0[y, x]])
+            want[..., 3] = np.nan
+            assert 0.3 < landed.mean() < 1.0, landed.mean()
+            assert same(got[view], want), (view, first_diff(got[view], want))
             continue
         prop = np.zeros_like(rnd)
         have = np.zeros((H, W), bool)
