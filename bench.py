@@ -358,9 +358,7 @@ def learned_section(dL, dR, gt, stream, cost_kind, cost_weight, plain_bad,
             "setup_ms": float(np.mean([e[0].elapsed_time(e[1]) for e in ev])),
             "iterations_ms": float(np.mean([e[1].elapsed_time(e[2])
                                             for e in ev]))}
-    matched = [n for n in runs if runs[n]["matched"]]
-    best = (max(matched, key=lambda n: runs[n]["value"]) if matched
-            else "jacobi")
+    best = fastest_matched(runs, "jacobi")
     views = runs[best]["seeded_views"]
     # the network alone as the chosen setup runs it: one pass over the
     # padded pair (and its mirror, as a batch of two, when both views are
@@ -494,6 +492,14 @@ def batched_section(stream, cost_kind, cost_weight, runs=2):
             "pixels": npix, "seed": BATCH_SEED}
 
 
+def fastest_matched(runs, fallback):
+    """the key of the fastest run (by "value") among those that matched
+    the plain run's quality ("matched"), or fallback when none did"""
+    matched = [n for n in runs if runs[n]["matched"]]
+    return max(matched, key=lambda n: runs[n]["value"]) if matched \
+        else fallback
+
+
 def matched_iterations(bad, target):
     """the fewest refine iterations k >= 1 whose bad-1.0px (bad[k]) is at
     most the target, or None"""
@@ -581,9 +587,7 @@ def serving_section(stream, cost_kind, cost_weight, runs=3):
             r["seeded_views"] = views
             r["bad_1px_pooled_by_refine_iterations"] = bad
             seeded[views] = r
-        ok = [v for v in seeded if seeded[v]["matched"]] or list(seeded)
-        out["learned"] = dict(seeded[max(ok, key=lambda v:
-                                         seeded[v]["value"])])
+        out["learned"] = dict(seeded[fastest_matched(seeded, 2)])
         out["learned"]["other_seeding"] = seeded[
             1 if out["learned"]["seeded_views"] == 2 else 2]
         tuning = b.tuning

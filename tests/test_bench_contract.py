@@ -50,3 +50,17 @@ def test_matched_iterations_is_the_fewest_that_reach_the_target():
     assert bench.matched_iterations(bad, 0.10) is None
     # k = 0 (the proposal alone) never counts as a refine count
     assert bench.matched_iterations([0.05, 0.2], 0.1) is None
+
+
+def test_fastest_matched_prefers_speed_among_matched_runs():
+    sys.path.insert(0, ROOT)
+    import bench
+    runs = {"a": {"value": 10.0, "matched": True},
+            "b": {"value": 30.0, "matched": False},
+            "c": {"value": 20.0, "matched": True}}
+    assert bench.fastest_matched(runs, "a") == "c"
+    # an unmatched run never wins, however fast
+    runs["c"]["matched"] = False
+    assert bench.fastest_matched(runs, "x") == "a"
+    runs["a"]["matched"] = False
+    assert bench.fastest_matched(runs, "x") == "x"
