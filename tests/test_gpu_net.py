@@ -296,3 +296,32 @@ def test_setup_proposal_bitexact(oracle, synth, views):
         want = np.where(keep[..., None], prop, rnd)
         assert 0.05 < keep.mean() < 0.95, keep.mean()
         assert same(got[view], want), (view, first_diff(got[view], want))
+
+
+def test_shipped_weights_match_plain_with_one_iteration(synth):
+    """The committed weights (cvt_amd/weights/proposal.bin) load, and on
+    the 720p bench pair (24 cells, disparity 0-128, 9x9 patch, the
+    colour+gradient cost) the proposal seeding the left view plus one
+    refine iteration reaches at least the bad-1px of plain PatchMatch
+    after 8 iterations (LR check at 1 px, the angle test off)."""
+    import os
+    from cvt_amd.net import WEIGHTS
+    assert os.path.exists(WEIGHTS)
+    L, R, gt = synth.make_pair(1280, 720, cells=24, dmax=128.0, channels=4,
+                               seed=0)
+    kw = dict(radius=4, disparity_max=128.0,
+              cost_kind=cvt_amd.PM_COST_COLOR_GRAD, cost_weight=25.5)
+    bad = {}
+    with cvt_amd.Context(**kw) as ctx:
+        ctx.set_images(L, R)
+        ctx.setup()
+        ctx.iterate(8)
+        d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 180.0)
+        bad["plain"] = cvt_amd.disparity.bad_pixel_rate(d, gt, 1.0)
+        with ProposalNet(weights=WEIGHTS) as net:
+            ctx.setup_proposal(net, 1)
+            ctx.iterate(1)
+            d = ctx.disparity(cvt_amd.DISPARITY_PP_LRCHECK, 1.0, 180.0)
+            bad["learned"] = cvt_amd.disparity.bad_pixel_rate(d, gt, 1.0)
+    print(bad, flush=True)
+    assert 0.0 < bad["learned"] <= bad["plain"] < 0.3, bad
