@@ -602,3 +602,27 @@ def test_results_do_not_depend_on_thread_count(oracle):
         assert int(threads) == n
         outs.append(digest)
     assert outs[0] == outs[1]
+
+
+def test_left_scatter_keys_name_their_source_columns(oracle):
+    """The right view's start of the left-only proposal seeding takes, at
+    right pixel x, viewprop of the left plane at W - 1 - key0, key0 the
+    first key the left view's scatter (lr = 1) put in slot x.  For a
+    fronto-parallel left field of disparity 3 that source is x + 3, and
+    viewprop maps its plane to one that sends x back to x + 3."""
+    O = oracle
+    W, H, d = 20, 2, 3.0
+    left = np.zeros((H, W, 4), F)
+    left[..., 0] = 1.0
+    left[..., 2] = -d                    # x' = x - 3 in the right view
+    vb = O.scatter(left, 1)
+    for y in range(H):
+        for x in range(W):
+            key0 = int(vb[y, x]) & 0xFFFFFFFF
+            if x + d >= W:               # no left pixel lands here
+                assert key0 == 0xFFFFFFFF, (y, x)
+                continue
+            src = W - 1 - key0
+            assert src == x + d, (y, x, src)
+            sr = O.viewprop(left[y, src])
+            assert O.transform(sr, x, y) == x + d
