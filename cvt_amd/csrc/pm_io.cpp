@@ -89,6 +89,18 @@ bool next_data_line(const PfmFile& f, size_t* pos,
   return false;
 }
 
+// A header size as String::toInteger reads it (cvt/util/String.h:510-513:
+// strtol with base 0, so 0x.. is hexadecimal and 0.. octal) and the
+// header's int field keeps it (PFMLoader.cpp:38-39,94-95): the low 32
+// bits as two's complement.
+int header_int(const std::string& tok) {
+  const long v = std::strtol(tok.c_str(), nullptr, 0);
+  const uint32_t low = (uint32_t)(unsigned long)v;
+  int out;
+  std::memcpy(&out, &low, sizeof out);
+  return out;
+}
+
 // PFMLoader::parseHeader (PFMLoader.cpp:68-105)
 int parse_pfm(const char* path, PfmFile* f) {
   if (!path) return io_fail(PM_ERR_ARG, "path is NULL");
@@ -109,8 +121,8 @@ int parse_pfm(const char* path, PfmFile* f) {
   if (!next_data_line(*f, &pos, &t) || t.size() < 2)
     return io_fail(PM_ERR_IO, "%s: PFM header corrupt, expected <width> "
                    "<height>", path);
-  f->W = std::atoi(t[0].c_str());
-  f->H = std::atoi(t[1].c_str());
+  f->W = header_int(t[0]);
+  f->H = header_int(t[1]);
   if (!next_data_line(*f, &pos, &t))
     return io_fail(PM_ERR_IO, "%s: PFM header corrupt, expected <scale>",
                    path);
@@ -118,10 +130,14 @@ int parse_pfm(const char* path, PfmFile* f) {
   if (f->W < 1 || f->H < 1)
     return io_fail(PM_ERR_IO, "%s: invalid PFM size %dx%d", path, f->W, f->H);
   f->data = pos;
-  const size_t need = (size_t)f->W * f->H * f->file_channels * sizeof(float);
-  if (f->bytes.size() - f->data < need)
-    return io_fail(PM_ERR_IO, "%s: not enough PFM data (%zu of %zu bytes)",
-                   path, f->bytes.size() - f->data, need);
+  // whole pixels present, compared row by row: W * H * channels * 4 can
+  // exceed size_t for header sizes up to INT_MAX and wrap to a small count
+  const size_t have = f->bytes.size() - f->data;
+  const size_t pixels = have / (sizeof(float) * f->file_channels);
+  if (pixels / (size_t)f->W < (size_t)f->H)
+    return io_fail(PM_ERR_IO, "%s: not enough PFM data (%zu bytes for %dx%d "
+                   "with %d channels)", path, have, f->W, f->H,
+                   f->file_channels);
   return PM_OK;
 }
 

@@ -58,6 +58,9 @@ def test_special_values_survive(tmp_path):
     (b"Pf\n2 2\n-1.0\n" + b"\0" * 15, "not enough PFM data"),
     (b"Pf\n0 2\n-1.0\n", "invalid PFM size"),
     (b"", "header corrupt"),
+    # W * H * 3 channels * 4 bytes wraps a 64-bit count to 11936
+    (b"PF\n2147380029 715862424\n-1.0\n" + b"\0" * 11936,
+     "not enough PFM data"),
 ])
 def test_malformed_files_are_errors(tmp_path, content, what):
     p = tmp_path / "bad.pfm"
@@ -71,6 +74,23 @@ def test_missing_file_is_an_error(tmp_path):
     with pytest.raises(D.IOError_) as e:
         pfm.load_pfm(tmp_path / "absent.pfm")
     assert "cannot open" in str(e.value)
+
+
+@pytest.mark.parametrize("sizes,shape", [
+    (b"0x3 02", (2, 3)),          # base 0: hexadecimal and octal
+    (b"010 1", (1, 8)),
+    (b"4294967299 1", (1, 3)),    # 2^32 + 3: the int keeps the low 32 bits
+])
+def test_header_sizes_read_as_the_reference_reads_them(tmp_path, sizes,
+                                                       shape):
+    """String::toInteger is strtol(s, NULL, 0) (cvt/util/String.h:510-513),
+    stored into PFMHeader's int fields (PFMLoader.cpp:38-39,94-95)."""
+    H, W = shape
+    data = np.arange(W * H, dtype="<f4")
+    p = tmp_path / "h.pfm"
+    p.write_bytes(b"Pf\n" + sizes + b"\n-1.0\n" + data.tobytes())
+    np.testing.assert_array_equal(pfm.load_pfm(p),
+                                  data.reshape(H, W)[::-1])
 
 
 def random_maps(seed, H=40, W=50):
