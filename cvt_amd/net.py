@@ -39,6 +39,42 @@ def padded(H, W):
     return (H + 7) // 8 * 8, (W + 7) // 8 * 8
 
 
+def _up(v, m):
+    return (v + m - 1) // m * m
+
+
+def param_count():
+    """floats of the network's parameter vector (pm_net_capi.cpp's layout):
+    per convolution its weights, output channels padded to 32 times 9 x
+    input channels padded to 32, then its 32-padded biases, each block
+    starting on a multiple of 64"""
+    off = 0
+    for cin, cout, _ in CONVS:
+        off = _up(off + _up(cout, 32) * _up(9 * cin, 32), 64)
+        off = _up(off + _up(cout, 32), 64)
+    return off
+
+
+def read_weights(path):
+    """The parameter vector of a weights file written by ``save``: the
+    8-byte magic, the parameter count (little-endian u64), then that many
+    little-endian floats.  ValueError for another file, another count or a
+    short one."""
+    with open(path, "rb") as f:
+        head = f.read(16)
+        if len(head) < 16 or head[:8] != _MAGIC:
+            raise ValueError("%s is not a proposal weights file" % path)
+        (n,) = struct.unpack("<Q", head[8:])
+        if n != param_count():
+            raise ValueError("%s holds %d parameters, the network %d"
+                             % (path, n, param_count()))
+        data = f.read(4 * n)
+    if len(data) != 4 * n:
+        raise ValueError("%s is truncated: %d of %d bytes of parameters"
+                         % (path, len(data), 4 * n))
+    return np.frombuffer(data, dtype="<f4").astype(np.float32)
+
+
 def forward_flops(H, W, N=1):
     """multiply-add work of one forward pass over N padded H x W inputs
     (2 FLOP per multiply-add, 3 x 3 taps; the head's 4 real outputs)"""
@@ -117,16 +153,7 @@ class ProposalNet:
             f.write(w.astype("<f4").tobytes())
 
     def load(self, path):
-        with open(path, "rb") as f:
-            head = f.read(16)
-            if head[:8] != _MAGIC:
-                raise ValueError("%s is not a proposal weights file" % path)
-            (n,) = struct.unpack("<Q", head[8:])
-            if n != self.num_params:
-                raise ValueError("%s holds %d parameters, the network %d"
-                                 % (path, n, self.num_params))
-            w = np.frombuffer(f.read(4 * n), dtype="<f4")
-        self.set_params(w)
+        self.set_params(read_weights(path))
 
     def forward(self, d_x, N, H, W, d_out):
         """d_x: N x H x W x 8 fp32 device pointer; d_out: N x H x W x 4."""
