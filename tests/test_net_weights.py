@@ -1,8 +1,9 @@
-"""The proposal network's weights file on the host (cvt_amd/net.py): the
-shipped file holds the parameter vector of the network's layout, and files
-of another kind, count or length are refused.  CPU only; loading the
-vector into a network (and the library's own count check) is covered by
-tests/test_gpu_net.py."""
+"""The proposal network's host side (cvt_amd/net.py): the shipped weights
+file holds the parameter vector of the network's layout, files of another
+kind, count or length are refused, and the layer table behind the
+parameter and FLOP counts follows the GPU tests' topology.  CPU only;
+loading the vector into a network (and the library's own count check) is
+covered by tests/test_gpu_net.py."""
 import struct
 
 import numpy as np
@@ -58,3 +59,27 @@ def test_short_header_is_refused(tmp_path):
     p.write_bytes(b"PMNETW1\0\x01")
     with pytest.raises(ValueError, match="not a proposal weights file"):
         net.read_weights(p)
+
+
+def test_layout_and_flops_follow_the_gpu_tests_topology():
+    """net.CONVS (behind the parameter count and the FLOP count the
+    network's roofline divides by) against the topology the GPU forward
+    test builds its torch reference from; that test's head stores 8
+    channels of which 4 are real outputs."""
+    from test_gpu_net import CH, CONVS as TOPOLOGY, layout
+    assert len(TOPOLOGY) == len(net.CONVS)
+    for (in1, in2, o, _, _), (cin, cout, lev) in zip(TOPOLOGY, net.CONVS):
+        assert cin == CH[in1][1] + (CH[in2][1] if in2 else 0)
+        assert cout == (4 if o == "out" else CH[o][1])
+        assert lev == CH[o][0]
+    assert layout()[1] == net.param_count()
+    # 2 FLOP per multiply-add, 9 taps, every output pixel of its level:
+    # the 720p pair and its mirror (padded to 720 x 1280)
+    macs = 0
+    for in1, in2, o, _, _ in TOPOLOGY:
+        lev = CH[o][0]
+        cin = CH[in1][1] + (CH[in2][1] if in2 else 0)
+        cout = 4 if o == "out" else CH[o][1]
+        macs += 9 * cin * cout * (720 >> lev) * (1280 >> lev)
+    assert net.forward_flops(720, 1280, N=2) == 2 * 2 * macs
+    assert round(net.forward_flops(720, 1280, N=1) / 1e9, 1) == 261.7
